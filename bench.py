@@ -1,0 +1,163 @@
+"""Benchmark of the Taylor-Green RK3 step on one MI355X.
+
+Workload: Taylor-Green vortex on a 512^3 grid, nu = 1/1600, dt = 5e-4, FP32
+storage and arithmetic with FP64 reductions (the chosen FP32 form); the same
+steps are also timed in FP64 (the plain FP64 path) and the dissipation of the
+two runs is compared after the timed steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+
+Prints one JSON line. `value` is Mpts/s = n^3 / (seconds per RK3 step) / 1e6.
+`roofline` describes the dominant pass kernel: algorithmic HBM bytes per launch
+divided by its mean launch time (HIP events on the solver's stream).
+`cpu_baseline` times the CPU oracle (a C++ restatement of the reference's CPU
+path) on one RK3 stage of the same 512^3 FP32 workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
+KERNEL_NAMES = ("k_inv_z", "k_lines", "k_inv_x", "k_fwd_x", "k_lines", "k_fwd_z")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def step_bytes(n: int, s: int) -> float:
+    """Algorithmic HBM bytes of one RK3 step: 36 R + 261 C."""
+    R = n ** 3 * s
+    C = n * n * (n // 2 + 1) * 2 * s
+    return 36.0 * R + 261.0 * C
+
+
+def time_steps(sph, n, precision, steps, warmup, dt):
+    s = sph.TaylorGreenSolver(n, 1600.0, precision)
+    for _ in range(warmup):
+        s.step_rk3(dt)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step_rk3(dt)
+    s.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    return s, sec
+
+
+def traffic_from_profiles(kernel: str, n: int):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{n}_fp32.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n: int, dt: float):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline: oracle {n}^3 FP32, one RK3 stage, {threads} threads")
+    o = oracle.TaylorGreenOracle(n, 1600.0, single=True)
+    sec = o.first_stage_seconds(dt)
+    o.close()
+    return {"value": round(n ** 3 / (3.0 * sec) / 1e6, 4), "unit": "Mpts/s", "cores": threads, "kind": "port",
+            "sample": f"one RK3 stage (of 3) of the {n}^3 FP32 Taylor-Green step, C2C oracle, {sec:.2f} s, "
+                      f"scaled x3 to a step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--reps", type=int, default=5, help="launches per pass for the kernel timing")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the workload is one {args.n}^3 grid per GPU and this "
+              f"build has no multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import spectral_solver_amd as sph
+    n, K, W = args.n, args.steps, args.warmup
+
+    log(f"fp32 {n}^3: warmup {W}, timing {K} steps")
+    s32, sec32 = time_steps(sph, n, "fp32", K, W, args.dt)
+    passes = s32.time_passes(args.reps)
+    log("pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
+
+    fp64 = None
+    eps_err = None
+    if not args.no_fp64:
+        log(f"fp64 {n}^3: warmup {W}, timing {K} steps")
+        s64, sec64 = time_steps(sph, n, "fp64", K, W, args.dt)
+        t = (W + K) * args.dt
+        d32, d64 = s32.diagnostics(t), s64.diagnostics(t)
+        eps_err = abs(d32.dissipation - d64.dissipation) / d64.dissipation
+        e_err = abs(d32.kinetic_energy - d64.kinetic_energy) / d64.kinetic_energy
+        z_err = abs(d32.enstrophy - d64.enstrophy) / d64.enstrophy
+        fp64 = {"ms_per_step": round(sec64 * 1e3, 4), "mpts_per_s": round(n ** 3 / sec64 / 1e6, 3),
+                "fp32_speedup": round(sec64 / sec32, 3),
+                "roofline_frac": round(step_bytes(n, 8) / sec64 / 1e9 / HBM_PEAK_GBS, 4)}
+        s64.close()
+
+    dom = max(range(6), key=lambda k: passes[PASS_NAMES[k]][0])
+    ms, by = passes[PASS_NAMES[dom]]
+    achieved = by / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(KERNEL_NAMES[dom], n),
+                "kernel": f"{PASS_NAMES[dom]} ({KERNEL_NAMES[dom]})", "launch_ms": round(ms, 4),
+                "algorithmic_bytes": by,
+                "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
+                "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()}}
+    s32.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(n, args.dt)
+        except Exception as e:  # reported, never fatal for the GPU number
+            log("cpu baseline failed:", e)
+
+    out = {
+        "metric": "ms/RK3 step & Mpts/s at 512^3, FP32 vs FP64, % HBM roofline, dissip. err",
+        "value": round(n ** 3 / sec32 / 1e6, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(sec32 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (analytic Taylor-Green initial condition)",
+        "config": {"workload": f"Taylor-Green vortex {n}^3, nu=1/1600, dt={args.dt}, FP32 fields, FP64 reductions",
+                   "grid": n, "dt": args.dt, "parallelism": "single GPU"},
+        "fp64": fp64,
+        "dissipation_rel_err_fp32_vs_fp64": eps_err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        out["energy_rel_err_fp32_vs_fp64"] = e_err
+        out["enstrophy_rel_err_fp32_vs_fp64"] = z_err
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * spectral_hip.h — C ABI of the MI355X (gfx950) pseudo-spectral Navier-Stokes
+ * RK3 hot path. Plain pointers, sizes and status codes only; no C++ or torch
+ * types cross this boundary.
+ *
+ * Every function returns 0 on success and a nonzero status on failure; the
+ * failure text ("what: detail", the reference's std::runtime_error messages)
+ * is then available from sph_last_error() on the calling thread.
+ *
+ * Layouts
+ *   physical field : real N^3 array, id = ix + nx*(iy + ny*iz) (x fastest),
+ *                    as in src/ava_taylor_green_impl.hpp:340-412 and
+ *                    src/fftw_fft.hpp:30-40.
+ *   spectral field : half spectrum (R2C), (nx/2+1)*ny*nz complex values,
+ *                    id = ky + ny*(kz + nz*kx) (ky fastest, kx slowest);
+ *                    mode (mx,my,mz) = (kx, signed(ky), signed(kz)) with the
+ *                    Nyquist index mapped to +n/2 (impl.hpp:123-125).
+ *                    Interleaved (re, im). Query the mode of every storage
+ *                    slot with sph_fft_spectral_modes().
+ *   precision      : SPH_FP32 or SPH_FP64 selects float or double storage
+ *                    and arithmetic (the reference's SPECTRAL_PRECISION,
+ *                    impl.hpp:48-52, becomes a run-time choice).
+ */
+#ifndef SPECTRAL_HIP_H
+#define SPECTRAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SPH_FP32 = 0, SPH_FP64 = 1 };
+
+typedef struct sph_fft_s* sph_fft_t;
+typedef struct sph_solver_s* sph_solver_t;
+
+/* Mirrors spectral::ava_solver::Diagnostics (src/ava_taylor_green.hpp:8-19). */
+typedef struct sph_diagnostics {
+    double t;
+    double kinetic_energy;
+    double dissipation;
+    double enstrophy;
+    double max_spectral_divergence;
+    double max_velocity;
+    double dt;
+    double cfl;
+    double main_dt;
+    double main_cfl;
+} sph_diagnostics;
+
+/* Mirrors the per-section timers printed by print_profile
+ * (src/ava_taylor_green_impl.hpp:1181-1286 and 563-594). Seconds. */
+typedef struct sph_profile {
+    double step;
+    double diagnostics;
+    double max_velocity;
+    long long step_calls;
+    long long rhs_calls;
+    long long diagnostics_calls;
+    long long max_velocity_calls;
+    long long forward_calls;  /* 3D forward transforms, counted as the reference counts them */
+    long long backward_calls; /* 3D backward transforms */
+} sph_profile;
+
+/* Thread-local text of the last failure on this thread ("" if none). */
+const char* sph_last_error(void);
+
+/* Number of visible GPUs (0 when no HIP device is present). */
+int sph_device_count(int* count);
+
+/* 1 if n is a transform length the library was compiled for, else 0. */
+int sph_supported_length(int n);
+
+/* ---------------------------------------------------------------------------
+ * FFT backend — replaces spectral::FftwFft3D / FlupsFft3D
+ * (src/fftw_fft.hpp:13-81, src/fftw_fft.cpp:8-87, src/flups_fft.hpp:28-133)
+ * and the internal seam detail::FftPlan (src/ava_taylor_green_impl.hpp:255-652).
+ * ------------------------------------------------------------------------- */
+
+/* FftwFft3D(shape, lengths, comm) (src/fftw_fft.cpp:8-36): shape is
+ * {nx, ny, nz}; every n must be > 1 and a supported length, every length > 0. */
+int sph_fft_create(const int shape[3], const double lengths[3], int precision, sph_fft_t* out);
+int sph_fft_destroy(sph_fft_t fft);
+
+/* physical_size() / spectral storage in complex values (fftw_fft.hpp:25-28). */
+int sph_fft_sizes(sph_fft_t fft, int64_t* physical_size, int64_t* spectral_modes);
+
+/* for_each_spectral(...) (fftw_fft.hpp:45-66) as a table: mode indices of
+ * every spectral storage slot, each array spectral_modes long. */
+int sph_fft_spectral_modes(sph_fft_t fft, int* mx, int* my, int* mz);
+
+/* Device-pointer transforms (FlupsFft3D::forward_device/backward_device,
+ * flups_fft.hpp:60-61; FftPlan::forward/backward, impl.hpp:414-561).
+ * Asynchronous on `stream` (a hipStream_t; NULL = the handle's own stream).
+ * forward: real physical -> half spectrum, unnormalised, sign e^{-i}.
+ * backward: half spectrum -> real physical, sign e^{+i}, multiplied by
+ * 1/(nx*ny*nz); the kx = 0 and kx = nx/2 planes contribute their real part,
+ * which is what Re(C2C backward) does (impl.hpp:555-561, fftw_fft.cpp:70-85).
+ * Buffers are caller-owned device memory in the handle's precision. */
+int sph_fft_forward(sph_fft_t fft, const void* physical, void* spectral, void* stream);
+int sph_fft_backward(sph_fft_t fft, const void* spectral, void* physical, void* stream);
+
+/* Host-vector transforms with the reference's std::vector<double> contract
+ * (fftw_fft.cpp:53-85): physical is physical_size doubles; spectral is
+ * 2*spectral_modes interleaved doubles. Synchronous. */
+int sph_fft_forward_host(sph_fft_t fft, const double* physical, double* spectral);
+int sph_fft_backward_host(sph_fft_t fft, const double* spectral, double* physical);
+
+/* ---------------------------------------------------------------------------
+ * Step API — replaces spectral::ava_solver::TaylorGreenSolver
+ * (src/ava_taylor_green.hpp:36-53, src/ava_taylor_green_impl.hpp:1289-1308).
+ * ------------------------------------------------------------------------- */
+
+/* TaylorGreenSolver(n, reynolds) (impl.hpp:703-716): allocates the state,
+ * sets the Taylor-Green field at cell centres, transforms and projects it.
+ * Fails with "grid size must be positive" / "Reynolds number must be positive". */
+int sph_solver_create(int n, double reynolds, int precision, sph_solver_t* out);
+int sph_solver_destroy(sph_solver_t solver);
+
+/* step_rk3(dt) (impl.hpp:979-1045): one Williamson low-storage RK3 step. */
+int sph_solver_step_rk3(sph_solver_t solver, double dt);
+
+/* diagnostics(t) (impl.hpp:1122-1179); dt/cfl fields are left 0. */
+int sph_solver_diagnostics(sph_solver_t solver, double t, sph_diagnostics* out);
+
+/* max_velocity() (impl.hpp:1105-1120). */
+int sph_solver_max_velocity(sph_solver_t solver, double* out);
+
+/* Sizes of the state: n, spectral modes per component (half spectrum). */
+int sph_solver_sizes(sph_solver_t solver, int* n, int64_t* spectral_modes);
+
+/* Copy component c (0..2) of u_hat to / from host memory as interleaved
+ * doubles in the spectral layout above (parity fixtures, checkpoints). */
+int sph_solver_get_uhat(sph_solver_t solver, int component, double* host);
+int sph_solver_set_uhat(sph_solver_t solver, int component, const double* host);
+
+/* Counters and timers behind print_profile (impl.hpp:1181-1286). */
+int sph_solver_profile(sph_solver_t solver, sph_profile* out);
+
+/* print_profile(rank, init, run) (impl.hpp:1181-1286): prints the
+ * profile_solver / profile_breakdown / profile_fft lines to stdout. */
+int sph_solver_print_profile(sph_solver_t solver, int rank, double init_time, double run_time);
+
+/* Capture the RK3 step into a hipGraph (1) or launch kernels directly (0).
+ * Default 1. Results are identical either way. */
+int sph_solver_set_graph(sph_solver_t solver, int enabled);
+
+/* Wait for all work queued by the solver. */
+int sph_solver_synchronize(sph_solver_t solver);
+
+/* Measure the pass kernels of one RK3 stage with HIP events on the solver's
+ * stream: each of the six passes is launched `reps` times back to back on the
+ * current state (the RK-update pass in a read-only timing form that writes a
+ * scratch copy) and its mean duration in milliseconds is written to ms[6] in
+ * the order inverse-z, inverse-y, inverse-x, forward-x, forward-y,
+ * forward-z(+update). Also returns the algorithmic HBM bytes per launch of
+ * each pass in bytes[6]. The state is unchanged. */
+int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
+
+/* run_taylor_green(config) (impl.hpp:1310-1410): fixed-dt or adaptive-CFL
+ * driver writing <output_dir>/taylor_green_diagnostics.csv. The fields mirror
+ * RunConfig (src/ava_taylor_green.hpp:21-34); precision selects FP32/FP64.
+ * final_row (may be NULL) receives the last diagnostics row. */
+typedef struct sph_run_config {
+    int n;
+    int steps;
+    int max_steps;
+    int output_every;
+    int print_every;
+    double dt;
+    double cfl;
+    double t_end;
+    double output_dt;
+    double reynolds;
+    int adaptive_cfl;
+    int precision;
+    const char* output_dir;
+} sph_run_config;
+
+int sph_run_taylor_green(const sph_run_config* config, sph_diagnostics* final_row);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECTRAL_HIP_H */

@@ -1,0 +1,187 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle restates the reference's AVA CPU path
+(src/ava_taylor_green_impl.hpp:658-1410, FFTW backend) in C++; see oracle.hpp.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, always as the checker of the GPU library, never as its fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_fft3d.argtypes = [ctypes.c_int] * 3 + [dp, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_solver_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_solver_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_solver_destroy.restype = None
+        L.orc_solver_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_solver_diagnostics.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.orc_solver_max_velocity.argtypes = [ctypes.c_void_p, dp]
+        L.orc_solver_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_solver_step_seconds.argtypes = [ctypes.c_void_p]
+        L.orc_solver_step_seconds.restype = ctypes.c_double
+        L.orc_solver_get_uhat.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.orc_solver_set_uhat.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.orc_run_taylor_green.argtypes = ([ctypes.c_int] * 6 + [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_char_p,
+                                           dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)])
+        L.orc_mode_tables.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3 + [
+            ctypes.POINTER(ctypes.c_ubyte), dp]
+        L.orc_dealiased_product.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.orc_solver_first_stage.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+DIAG_KEYS = ("t", "kinetic_energy", "dissipation", "enstrophy", "max_spectral_divergence",
+             "max_velocity", "dt", "cfl", "main_dt", "main_cfl")
+COUNTER_KEYS = ("forward_calls", "backward_calls", "step_calls", "rhs_calls",
+                "diagnostics_calls", "max_velocity_calls")
+
+
+def fft3d(a: np.ndarray, sign: int, single: bool = False) -> np.ndarray:
+    """Unnormalised C2C transform of a complex [nz, ny, nx] array."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    nz, ny, nx = a.shape
+    out = np.empty_like(a)
+    _check(lib().orc_fft3d(nx, ny, nz, _dptr(a.view(np.float64)), _dptr(out.view(np.float64)), sign,
+                           1 if single else 0))
+    return out
+
+
+class TaylorGreenOracle:
+    """Host restatement of TaylorGreenSolver (C2C storage, x fastest)."""
+
+    def __init__(self, n: int, reynolds: float, single: bool = False, initialise: bool = True):
+        self.n = n
+        self.single = single
+        h = ctypes.c_void_p()
+        _check(lib().orc_solver_create(n, reynolds, 1 if single else 0, 1 if initialise else 0, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib().orc_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step_rk3(self, dt: float) -> None:
+        _check(lib().orc_solver_step(self._h, dt))
+
+    def diagnostics(self, t: float) -> dict:
+        out = np.zeros(10)
+        _check(lib().orc_solver_diagnostics(self._h, t, _dptr(out)))
+        return dict(zip(DIAG_KEYS, out.tolist()))
+
+    def max_velocity(self) -> float:
+        v = ctypes.c_double()
+        _check(lib().orc_solver_max_velocity(self._h, ctypes.byref(v)))
+        return v.value
+
+    def counters(self) -> dict:
+        out = (ctypes.c_longlong * 6)()
+        _check(lib().orc_solver_counters(self._h, out))
+        return dict(zip(COUNTER_KEYS, list(out)))
+
+    def first_stage_seconds(self, dt: float) -> float:
+        """Run only RK3 stage 1 and return its wall time (CPU-baseline sample)."""
+        sec = ctypes.c_double()
+        _check(lib().orc_solver_first_stage(self._h, dt, ctypes.byref(sec)))
+        return sec.value
+
+    def step_seconds(self) -> float:
+        return lib().orc_solver_step_seconds(self._h)
+
+    def u_hat(self, c: int) -> np.ndarray:
+        """Component c of the C2C state as complex [nz, ny, nx]."""
+        n = self.n
+        out = np.empty((n, n, n), np.complex128)
+        _check(lib().orc_solver_get_uhat(self._h, c, _dptr(out.view(np.float64))))
+        return out
+
+    def set_u_hat(self, c: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        _check(lib().orc_solver_set_uhat(self._h, c, _dptr(a.view(np.float64))))
+
+
+def run_taylor_green(n=64, steps=80, max_steps=10000000, output_every=1, print_every=10, adaptive_cfl=False,
+                     dt=0.0025, cfl=0.4, t_end=None, output_dt=0.1, reynolds=1600.0, single=False,
+                     output_dir="output_oracle"):
+    """Driver restatement; returns (final_row, counters, steps_taken)."""
+    if t_end is None:
+        t_end = steps * dt
+    row = np.zeros(10)
+    cnt = (ctypes.c_longlong * 6)()
+    st = ctypes.c_int()
+    _check(lib().orc_run_taylor_green(n, steps, max_steps, output_every, print_every, 1 if adaptive_cfl else 0,
+                                      dt, cfl, t_end, output_dt, reynolds, 1 if single else 0,
+                                      output_dir.encode(), _dptr(row), cnt, ctypes.byref(st)))
+    return dict(zip(DIAG_KEYS, row.tolist())), dict(zip(COUNTER_KEYS, list(cnt))), st.value
+
+
+def mode_tables(n: int):
+    """(mx, my, mz, keep, phase) of the reference's metadata, shaped [nz, ny, nx]."""
+    t = n * n * n
+    mx, my, mz = (np.empty(t, np.int32) for _ in range(3))
+    keep = np.empty(t, np.uint8)
+    ph = np.empty(t, np.complex128)
+    ip = ctypes.POINTER(ctypes.c_int)
+    _check(lib().orc_mode_tables(n, mx.ctypes.data_as(ip), my.ctypes.data_as(ip), mz.ctypes.data_as(ip),
+                                 keep.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _dptr(ph.view(np.float64))))
+    shp = (n, n, n)
+    return mx.reshape(shp), my.reshape(shp), mz.reshape(shp), keep.reshape(shp).astype(bool), ph.reshape(shp)
+
+
+def dealiased_product(a_hat: np.ndarray, b_hat: np.ndarray) -> np.ndarray:
+    """Dealiased product spectrum (one accumulate_product combine) of two C2C spectra."""
+    n = a_hat.shape[0]
+    a = np.ascontiguousarray(a_hat, np.complex128)
+    b = np.ascontiguousarray(b_hat, np.complex128)
+    out = np.empty_like(a)
+    _check(lib().orc_dealiased_product(n, _dptr(a.view(np.float64)), _dptr(b.view(np.float64)),
+                                       _dptr(out.view(np.float64))))
+    return out
+
+
+def c2c_to_half(a: np.ndarray) -> np.ndarray:
+    """Restrict a C2C [kz, ky, kx] spectrum to the half-spectrum layout
+    [kx in 0..n/2][kz][ky] used by the GPU library."""
+    nx = a.shape[2]
+    return np.ascontiguousarray(np.transpose(a[:, :, : nx // 2 + 1], (2, 0, 1)))

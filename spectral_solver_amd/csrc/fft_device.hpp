@@ -1,0 +1,235 @@
+// Device building blocks of the batched 1D FFT used by every pass kernel.
+//
+// A work-group owns a "tile" of NL lines of length N held in LDS with layout
+// [element][line] (line fastest, row stride NL+1 complex values so that both
+// the line-fast stage accesses and the element-fast global staging are free
+// of systematic bank conflicts). Each Stockham stage of radix R reads R values
+// per butterfly task into registers, applies twiddles and a register DFT_R,
+// then writes them back in self-sorting (Stockham) order:
+//   x_r = a[j + r*N/R] * w_{Ns*R}^{r*(j mod Ns)},  y = DFT_R(x),
+//   a'[(j - j mod Ns)*R + (j mod Ns) + r*Ns] = y_r.
+// Butterfly tasks are numbered line-fast, so a wavefront touches consecutive
+// LDS addresses in every stage. Twiddles come from a per-length table
+// e^{-2*pi*i*m/N}, computed on the host in long double and rounded once to
+// the storage type; inverse transforms use the conjugate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sph {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T x;
+    T y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, T s) { return {s * a.x, s * a.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
+
+// ---------------------------------------------------------------------------
+// Radix plan: powers of two as radix-8 stages (a trailing 8*2 becomes 4*4),
+// then 3s and 5s.
+// ---------------------------------------------------------------------------
+struct RadixPlan {
+    int r[16];
+    int count;
+};
+
+constexpr RadixPlan radix_plan(int n) {
+    RadixPlan p{};
+    int twos = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        ++twos;
+    }
+    int eights = twos / 3;
+    const int rem = twos % 3;
+    int fours = 0, pairs = 0;
+    if (rem == 1 && eights > 0) {
+        eights -= 1;
+        fours = 2;
+    } else if (rem == 1) {
+        pairs = 1;
+    } else if (rem == 2) {
+        fours = 1;
+    }
+    for (int i = 0; i < eights; ++i) p.r[p.count++] = 8;
+    for (int i = 0; i < fours; ++i) p.r[p.count++] = 4;
+    for (int i = 0; i < pairs; ++i) p.r[p.count++] = 2;
+    while (n % 3 == 0) {
+        n /= 3;
+        p.r[p.count++] = 3;
+    }
+    while (n % 5 == 0) {
+        n /= 5;
+        p.r[p.count++] = 5;
+    }
+    if (n != 1) p.count = -1;  // unsupported factor
+    return p;
+}
+
+constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 0; }
+
+// ---------------------------------------------------------------------------
+// Tile geometry per (storage type, length).
+// ---------------------------------------------------------------------------
+constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <typename T, int N>
+struct Tile {
+    // lines per tile: one 128-byte row segment of complex values per element
+    static constexpr int NL = sizeof(T) == 4 ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8);
+    static constexpr int STRIDE = NL + 1;
+    static constexpr int ELEMS = NL * N;
+    static constexpr int NT = clampi(round_up(ELEMS / 8, 64), 64, 1024);
+    static constexpr int E = (ELEMS + NT - 1) / NT;  // elements per thread
+    static constexpr int LDS = STRIDE * N;           // complex values in the tile buffer
+};
+
+// ---------------------------------------------------------------------------
+// Register DFTs. rot() multiplies by -i (forward) or +i (inverse).
+// ---------------------------------------------------------------------------
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> rot(cx<T> a) {
+    if constexpr (INV) {
+        return {-a.y, a.x};
+    } else {
+        return {a.y, -a.x};
+    }
+}
+
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void dft(cx<T>* v) {
+    if constexpr (R == 2) {
+        const cx<T> a = v[0];
+        v[0] = a + v[1];
+        v[1] = a - v[1];
+    } else if constexpr (R == 4) {
+        const cx<T> a = v[0] + v[2], b = v[0] - v[2];
+        const cx<T> c = v[1] + v[3], d = rot<INV>(v[1] - v[3]);
+        v[0] = a + c;
+        v[2] = a - c;
+        v[1] = b + d;
+        v[3] = b - d;
+    } else if constexpr (R == 8) {
+        const T h = T(0.707106781186547524400844362104849039L);
+        const cx<T> a0 = v[0] + v[4], a1 = v[0] - v[4];
+        const cx<T> a2 = v[2] + v[6], a3 = rot<INV>(v[2] - v[6]);
+        const cx<T> a4 = v[1] + v[5], a5 = v[1] - v[5];
+        const cx<T> a6 = v[3] + v[7], a7 = rot<INV>(v[3] - v[7]);
+        const cx<T> e0 = a0 + a2, e2 = a0 - a2, e1 = a1 + a3, e3 = a1 - a3;
+        const cx<T> o0 = a4 + a6, o2 = a4 - a6, o1 = a5 + a7, o3 = a5 - a7;
+        const cx<T> t1 = (o1 + rot<INV>(o1)) * h;
+        const cx<T> t2 = rot<INV>(o2);
+        const cx<T> t3 = (rot<INV>(o3) - o3) * h;
+        v[0] = e0 + o0;
+        v[4] = e0 - o0;
+        v[1] = e1 + t1;
+        v[5] = e1 - t1;
+        v[2] = e2 + t2;
+        v[6] = e2 - t2;
+        v[3] = e3 + t3;
+        v[7] = e3 - t3;
+    } else if constexpr (R == 3) {
+        const T s = T(0.866025403784438646763723170752936183L);
+        const cx<T> t = v[1] + v[2];
+        const cx<T> m = v[0] - t * T(0.5);
+        const cx<T> d = rot<INV>((v[1] - v[2]) * s);
+        v[0] = v[0] + t;
+        v[1] = m + d;
+        v[2] = m - d;
+    } else if constexpr (R == 5) {
+        // y_k = sum_q x_q w^{qk}, w = e^{-+2 pi i/5}
+        const T c1 = T(0.309016994374947424102293417182819059L);   // cos(2pi/5)
+        const T c2 = T(-0.809016994374947424102293417182819059L);  // cos(4pi/5)
+        const T s1 = T(0.951056516295153572116439333379382143L);   // sin(2pi/5)
+        const T s2 = T(0.587785252292473129168705954639072769L);   // sin(4pi/5)
+        const cx<T> p1 = v[1] + v[4], m1 = v[1] - v[4];
+        const cx<T> p2 = v[2] + v[3], m2 = v[2] - v[3];
+        const cx<T> a1 = v[0] + p1 * c1 + p2 * c2;
+        const cx<T> a2 = v[0] + p1 * c2 + p2 * c1;
+        const cx<T> b1 = rot<INV>(m1 * s1 + m2 * s2);
+        const cx<T> b2 = rot<INV>(m1 * s2 - m2 * s1);
+        v[0] = v[0] + p1 + p2;
+        v[1] = a1 + b1;
+        v[4] = a1 - b1;
+        v[2] = a2 + b2;
+        v[3] = a2 - b2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One in-place Stockham stage over the whole tile.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int NL, int NT, int R, int NS, bool INV>
+__device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* twl, int tid) {
+    constexpr int M = N / R;
+    constexpr int TASKS = NL * M;
+    constexpr int TPT = (TASKS + NT - 1) / NT;
+    constexpr int STRIDE = NL + 1;
+    cx<T> v[TPT][R];
+#pragma unroll
+    for (int i = 0; i < TPT; ++i) {
+        const int t = tid + i * NT;
+        if (TASKS % NT == 0 || t < TASKS) {
+            const int line = t % NL;
+            const int j = t / NL;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * M) * STRIDE + line];
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = twl[r * k * (N / (NS * R))];
+                    if constexpr (INV) w.y = -w.y;
+                    v[i][r] = cmul(v[i][r], w);
+                }
+            }
+            dft<R, INV>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TPT; ++i) {
+        const int t = tid + i * NT;
+        if (TASKS % NT == 0 || t < TASKS) {
+            const int line = t % NL;
+            const int j = t / NL;
+            const int k = j % NS;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(base + r * NS) * STRIDE + line] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int NL, int NT, bool INV, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) {
+    constexpr RadixPlan P = radix_plan(N);
+    if constexpr (S < P.count) {
+        constexpr int R = P.r[S];
+        fft_stage<T, N, NL, NT, R, NS, INV>(buf, twl, tid);
+        fft_tile<T, N, NL, NT, INV, S + 1, NS * R>(buf, twl, tid);
+    }
+}
+
+template <typename T, int N, int NT>
+__device__ __forceinline__ void load_twiddles(cx<T>* twl, const cx<T>* __restrict__ tw, int tid) {
+#pragma unroll
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+}
+
+}  // namespace sph

@@ -1,0 +1,71 @@
+// Host-side dispatch of the pass kernels: one table of launchers per
+// (storage type, transform length). The kernels themselves are compiled in
+// passes_f32.hip / passes_f64.hip.
+#pragma once
+
+#include "passes.hpp"
+
+namespace sph {
+
+template <typename T>
+struct PassTable {
+    int n;   // transform length
+    int nl;  // lines per tile
+    int nt;  // threads per work-group
+    void (*inv_z)(const InvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
+    void (*lines)(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t);
+    void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
+    void (*fwd_x)(const FwdXArgs<T>&, int mode, int ny, int nz, hipStream_t);
+    void (*fwd_z)(const FwdZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
+};
+
+// nullptr when n is not a compiled length.
+template <typename T>
+const PassTable<T>* pass_table(int n);
+
+// The compiled transform lengths.
+#define SPH_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
+
+template <typename T, int N>
+struct Launch {
+    using TL = Tile<T, N>;
+    static void inv_z(const InvZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
+        const dim3 grid((nky + TL::NL - 1) / TL::NL, nkx);
+        if (mode == IZ_RHS)
+            k_inv_z<T, N, IZ_RHS><<<grid, TL::NT, 0, s>>>(a);
+        else if (mode == IZ_DIAG)
+            k_inv_z<T, N, IZ_DIAG><<<grid, TL::NT, 0, s>>>(a);
+        else
+            k_inv_z<T, N, IZ_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static void lines(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t s) {
+        const dim3 grid(static_cast<unsigned>((nlines + TL::NL - 1) / TL::NL));
+        if (inverse)
+            k_lines<T, N, true><<<grid, TL::NT, 0, s>>>(data, nlines, tw);
+        else
+            k_lines<T, N, false><<<grid, TL::NT, 0, s>>>(data, nlines, tw);
+    }
+    static void inv_x(const InvXArgs<T>& a, int ny, int nz, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+        k_inv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static void fwd_x(const FwdXArgs<T>& a, int mode, int ny, int nz, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+        if (mode == FX_PRODUCTS)
+            k_fwd_x<T, N, FX_PRODUCTS><<<grid, TL::NT, 0, s>>>(a);
+        else
+            k_fwd_x<T, N, FX_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static void fwd_z(const FwdZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
+        const dim3 grid((nky + TL::NL - 1) / TL::NL, nkx);
+        if (mode == FZ_RK)
+            k_fwd_z<T, N, FZ_RK><<<grid, TL::NT, 0, s>>>(a);
+        else if (mode == FZ_PROJECT)
+            k_fwd_z<T, N, FZ_PROJECT><<<grid, TL::NT, 0, s>>>(a);
+        else
+            k_fwd_z<T, N, FZ_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static constexpr PassTable<T> table{N, TL::NL, TL::NT, &inv_z, &lines, &inv_x, &fwd_x, &fwd_z};
+};
+
+}  // namespace sph

@@ -1,0 +1,86 @@
+"""Host-side checks of the C ABI (no GPU needed, no compute calls).
+
+The library must load, export every function include/spectral_hip.h declares,
+and reject invalid input with the reference's messages before touching a device
+(src/ava_taylor_green_impl.hpp:704-709, src/fftw_fft.cpp:12-24).
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+import spectral_solver_amd as sph
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "spectral_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sph.load_library()
+    names = declared_functions()
+    assert len(names) >= 20
+    bound = {s[0] for s in sph.SYMBOLS}
+    for name in names:
+        assert hasattr(lib, name), f"{name} not exported"
+        assert name in bound, f"{name} has no Python binding"
+
+
+def test_supported_lengths():
+    lib = sph.load_library()
+    for n in (8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024):
+        assert lib.sph_supported_length(n) == 1
+    for n in (0, 1, 7, 100, 2048):
+        assert lib.sph_supported_length(n) == 0
+
+
+def _solver_error(n, re_, prec=sph.FP64):
+    lib = sph.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.sph_solver_create(n, re_, prec, ctypes.byref(h))
+    assert rc != 0
+    return lib.sph_last_error().decode()
+
+
+def test_solver_argument_errors_match_reference():
+    assert _solver_error(0, 1600.0) == "grid size must be positive"
+    assert _solver_error(-4, 1600.0) == "grid size must be positive"
+    assert _solver_error(64, 0.0) == "Reynolds number must be positive"
+    assert _solver_error(64, -1.0) == "Reynolds number must be positive"
+    assert "unsupported transform length" in _solver_error(100, 1600.0)
+    assert "precision" in _solver_error(64, 1600.0, 7)
+
+
+def test_fft_argument_errors_match_reference():
+    with pytest.raises(RuntimeError, match="all grid sizes must be greater than one"):
+        sph.HipFft3D((1, 8, 8), (1.0, 1.0, 1.0))
+    with pytest.raises(RuntimeError, match="all domain lengths must be positive"):
+        sph.HipFft3D((8, 8, 8), (1.0, 0.0, 1.0))
+    with pytest.raises(RuntimeError, match="unsupported transform length"):
+        sph.HipFft3D((8, 8, 100), (1.0, 1.0, 1.0))
+
+
+def test_driver_argument_errors_match_reference():
+    with pytest.raises(RuntimeError, match="--dt, --cfl, and --output-dt must be positive"):
+        sph.run_taylor_green(sph.RunConfig(n=16, dt=0.0))
+    with pytest.raises(RuntimeError, match="--dt, --cfl, and --output-dt must be positive"):
+        sph.run_taylor_green(sph.RunConfig(n=16, cfl=-1.0))
+
+
+def test_null_handles_are_rejected():
+    lib = sph.load_library()
+    assert lib.sph_solver_step_rk3(None, 1e-3) != 0
+    assert "null handle" in lib.sph_last_error().decode()
+    assert lib.sph_fft_forward(None, None, None, None) != 0
+
+
+@pytest.mark.skipif(sph.device_count() > 0, reason="checks the no-GPU failure path")
+def test_no_device_fails_loudly():
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        sph.TaylorGreenSolver(16, 1600.0)

@@ -1,0 +1,120 @@
+"""GPU parity of the HIP real<->complex 3D transforms against the oracle.
+
+The oracle's C2C transform (pinned in test_oracle.py) restricted to
+kx <= nx/2 is the expected half spectrum. For the backward transform the
+expectation is Re(C2C backward)/N of the conjugate-symmetric completion of the
+half spectrum, which keeps the reference's treatment of the kx = 0 and
+kx = nx/2 planes (their values enter through Re, impl.hpp:555-561).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import spectral_solver_amd as sph
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (12, 12, 12), (32, 16, 8), (8, 24, 48), (64, 64, 64), (96, 32, 16), (128, 8, 12)]
+
+
+def half_of(c2c):
+    return oracle.c2c_to_half(c2c)
+
+
+def completion(half, shape):
+    """C2C [z, y, x] spectrum from a half spectrum [kx][kz][ky]."""
+    nx, ny, nz = shape
+    full = np.zeros((nz, ny, nx), np.complex128)
+    full[:, :, : nx // 2 + 1] = np.transpose(half, (1, 2, 0))
+    for kx in range(nx // 2 + 1, nx):
+        src = np.transpose(half[nx - kx], (0, 1))  # [kz][ky]
+        full[:, :, kx] = np.conj(np.roll(np.flip(src, (0, 1)), 1, axis=(0, 1)))
+    return full
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_matches_oracle_fp64(shape):
+    print("case forward fp64", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((nz, ny, nx))
+    fft = sph.HipFft3D(shape, (2 * np.pi,) * 3, precision="fp64")
+    got = fft.forward(u.reshape(-1)).view(np.complex128).reshape(fft.half_shape())
+    ref = half_of(oracle.fft3d(u, -1))
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_backward_matches_oracle_fp64(shape):
+    print("case backward fp64", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(12)
+    H = nx // 2 + 1
+    half = rng.standard_normal((H, nz, ny)) + 1j * rng.standard_normal((H, nz, ny))
+    fft = sph.HipFft3D(shape, (1.0, 2.0, 3.0), precision="fp64")
+    got = fft.backward(half.view(np.float64).reshape(-1)).reshape(nz, ny, nx)
+    ref = oracle.fft3d(completion(half, shape), 1).real / (nx * ny * nz)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64, 64), (256, 256, 256)])
+def test_forward_fp32_error(shape):
+    print("case forward fp32", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(13)
+    u = rng.standard_normal((nz, ny, nx))
+    f32 = sph.HipFft3D(shape, (1.0, 1.0, 1.0), precision="fp32")
+    f64 = sph.HipFft3D(shape, (1.0, 1.0, 1.0), precision="fp64")
+    a = f32.forward(u.reshape(-1).astype(np.float32).astype(np.float64))
+    b = f64.forward(u.reshape(-1).astype(np.float32).astype(np.float64))
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel < 2e-6, rel
+
+
+def test_reference_roundtrip_known_answer():
+    # cpp_tests/spectral_ns_tests.cpp:16-32 on the 12^3 grid
+    n = 12
+    x = (np.arange(n) + 0.5) * 2 * np.pi / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = (np.sin(X) + 0.25 * np.cos(Y)).reshape(-1)
+    fft = sph.HipFft3D((n, n, n), (2 * np.pi,) * 3)
+    assert np.abs(fft.backward(fft.forward(u)) - u).max() < 1e-10
+
+
+def test_golden_numpy_vectors_forward():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
+    for key in g.files:
+        if not key.startswith("in_"):
+            continue
+        a = g[key].real  # real input; its forward spectrum is fft of the real part
+        nz, ny, nx = a.shape
+        fft = sph.HipFft3D((nx, ny, nz), (1.0, 1.0, 1.0))
+        got = fft.forward(a.reshape(-1)).view(np.complex128).reshape(fft.half_shape())
+        ref = half_of(np.fft.fftn(a))
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), key
+
+
+def test_spectral_mode_table():
+    fft = sph.HipFft3D((8, 12, 16), (1.0, 1.0, 1.0))
+    assert fft.spectral_storage_size() == 2 * 5 * 12 * 16
+    assert fft.mx.max() == 4 and fft.my.max() == 6 and fft.my.min() == -5 and fft.mz.max() == 8
+    seen = []
+    fft.for_each_spectral(lambda i, m: seen.append((i, m.mx, m.my, m.mz)) if len(seen) < 3 else None)
+    assert seen[0] == (0, 0, 0, 0) and seen[1] == (2, 0, 1, 0)
+
+
+def test_device_pointer_transforms():
+    torch = pytest.importorskip("torch")
+    n = 32
+    fft = sph.HipFft3D((n, n, n), (1.0, 1.0, 1.0), precision="fp64")
+    u = torch.randn(n, n, n, dtype=torch.float64, device="cuda")
+    spec = torch.empty((n // 2 + 1) * n * n * 2, dtype=torch.float64, device="cuda")
+    back = torch.empty_like(u)
+    torch.cuda.synchronize()
+    fft.forward_device(u.data_ptr(), spec.data_ptr())
+    fft.backward_device(spec.data_ptr(), back.data_ptr())
+    torch.cuda.synchronize()
+    ref = fft.forward(u.cpu().numpy().reshape(-1))
+    assert np.abs(spec.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert (back - u).abs().max().item() < 1e-12

@@ -1,0 +1,182 @@
+"""GPU parity of the fused RK3 step against the CPU oracle (FP64 to 1e-12
+relative L2), the reference's published end state, and the FP32 form's error.
+
+The oracle stores the C2C state; its restriction to kx <= n/2 is the GPU's
+half-spectrum state (test_gpu_fft.py covers the Re() rule on the kx = 0 and
+kx = n/2 planes).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import spectral_solver_amd as sph
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def state_error(gpu, orc):
+    num = sum(np.linalg.norm(gpu.u_hat(c) - oracle.c2c_to_half(orc.u_hat(c))) ** 2 for c in range(3))
+    den = sum(np.linalg.norm(oracle.c2c_to_half(orc.u_hat(c))) ** 2 for c in range(3))
+    return np.sqrt(num / den)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_initial_condition_matches_oracle(n):
+    print("case ic", n, flush=True)
+    g = sph.TaylorGreenSolver(n, 1600.0, "fp64")
+    o = oracle.TaylorGreenOracle(n, 1600.0)
+    assert state_error(g, o) < 1e-14
+    dg, do = g.diagnostics(0.0), o.diagnostics(0.0)
+    for k in ("kinetic_energy", "dissipation", "enstrophy", "max_velocity"):
+        assert abs(getattr(dg, k) - do[k]) <= 1e-13 * abs(do[k]), k
+    assert dg.max_spectral_divergence < 1e-8
+
+
+def test_golden_trajectory_fixture():
+    g0 = np.load(os.path.join(ROOT, "tests", "golden", "tg16_fp64.npz"))
+    s = sph.TaylorGreenSolver(16, float(g0["reynolds"]), "fp64")
+    for k in range(int(g0["steps"])):
+        s.step_rk3(float(g0["dt"]))
+        d = s.diagnostics((k + 1) * float(g0["dt"]))
+        ref = g0["diagnostics"][k + 1]
+        assert abs(d.kinetic_energy - ref[1]) <= 1e-13 * ref[1]
+        assert abs(d.dissipation - ref[2]) <= 1e-12 * ref[2]
+    for c in range(3):
+        assert rel_l2(s.u_hat(c), g0[f"u{c}"]) < 1e-12
+
+
+def random_state(n, seed):
+    """Random solenoidal, truncated C2C state with non-Hermitian values on the
+    three kept axis-Nyquist modes (which the reference evolves)."""
+    rng = np.random.default_rng(seed)
+    o = oracle.TaylorGreenOracle(n, 400.0, initialise=False)
+    u = [oracle.fft3d(rng.standard_normal((n, n, n)) * 0.3, -1) for _ in range(3)]
+    mx, my, mz, keep, _ = oracle.mode_tables(n)
+    k2 = (mx ** 2 + my ** 2 + mz ** 2).astype(float)
+    ok = keep & (k2 > 0)
+    dot = (mx * u[0] + my * u[1] + mz * u[2]) / np.where(k2 > 0, k2, 1)
+    u = [np.where(ok, u[c] - [mx, my, mz][c] * dot, 0) for c in range(3)]
+    h = n // 2
+    for (iz, iy, ix) in ((0, 0, h), (0, h, 0), (h, 0, 0)):
+        for c in range(3):
+            if [ix, iy, iz][c] == 0:  # keep it divergence free
+                u[c][iz, iy, ix] += (0.2 + 0.9j) * n ** 3 * 0.01
+    for c in range(3):
+        o.set_u_hat(c, u[c])
+    return o, u
+
+
+@pytest.mark.parametrize("n,steps", [(32, 3), (48, 2), (24, 2)])
+def test_random_state_with_nyquist_modes_matches_oracle(n, steps):
+    print("case random", n, flush=True)
+    o, u = random_state(n, 5 + n)
+    g = sph.TaylorGreenSolver(n, 400.0, "fp64")
+    for c in range(3):
+        g.set_u_hat(c, oracle.c2c_to_half(u[c]))
+    for _ in range(steps):
+        o.step_rk3(2e-3)
+        g.step_rk3(2e-3)
+    err = state_error(g, o)
+    assert err < 1e-12, err
+    d1, d2 = g.diagnostics(0.0), o.diagnostics(0.0)
+    assert abs(d1.kinetic_energy - d2["kinetic_energy"]) <= 1e-12 * d2["kinetic_energy"]
+    assert abs(d1.enstrophy - d2["enstrophy"]) <= 1e-12 * d2["enstrophy"]
+
+
+def test_config_64cube_fp64_200_steps_matches_oracle():
+    """64^3 TG, nu = 1/1600, dt = 1e-3, 200 fixed steps, FP64: per-step E and
+    dissipation, and the fields at the end, against the oracle."""
+    n, dt, steps = 64, 1e-3, 200
+    g = sph.TaylorGreenSolver(n, 1600.0, "fp64")
+    o = oracle.TaylorGreenOracle(n, 1600.0)
+    worst_e = worst_eps = 0.0
+    for k in range(steps):
+        g.step_rk3(dt)
+        o.step_rk3(dt)
+        if k % 10 == 9 or k == steps - 1:
+            dg, do = g.diagnostics((k + 1) * dt), o.diagnostics((k + 1) * dt)
+            worst_e = max(worst_e, abs(dg.kinetic_energy - do["kinetic_energy"]) / do["kinetic_energy"])
+            worst_eps = max(worst_eps, abs(dg.dissipation - do["dissipation"]) / do["dissipation"])
+    print("worst rel E", worst_e, "worst rel eps", worst_eps, flush=True)
+    assert worst_e < 1e-12 and worst_eps < 1e-12
+    err = state_error(g, o)
+    print("field rel L2", err, flush=True)
+    assert err < 1e-12
+
+
+def test_published_64cube_fp64_end_state_at_t20(tmp_path):
+    """Adaptive-CFL driver to t = 20 (Benchmark.md:584-591, 352 steps)."""
+    cfg = sph.RunConfig(n=64, adaptive_cfl=True, dt=0.5, cfl=0.5, t_end=20.0, output_dt=1.0, reynolds=1600.0,
+                        output_dir=str(tmp_path), precision="fp64", print_every=100)
+    d = sph.run_taylor_green(cfg)
+    assert abs(d.t - 20.0) < 1e-12
+    assert abs(d.kinetic_energy - 0.01687796507292055) / 0.01687796507292055 < 1e-11
+    assert abs(d.dissipation - 0.0012350569438618611) / 0.0012350569438618611 < 1e-11
+    assert abs(d.enstrophy - 0.98804555508948877) / 0.98804555508948877 < 1e-11
+    assert abs(d.max_velocity - 0.48555765714990651) < 1e-10
+    rows = open(os.path.join(str(tmp_path), "taylor_green_diagnostics.csv")).read().strip().splitlines()
+    assert len(rows) == 22  # header + t = 0..20
+
+
+def test_fp32_form_error_against_fp64():
+    n, dt, steps = 64, 1e-3, 100
+    g32 = sph.TaylorGreenSolver(n, 1600.0, "fp32")
+    g64 = sph.TaylorGreenSolver(n, 1600.0, "fp64")
+    worst = 0.0
+    for k in range(steps):
+        g32.step_rk3(dt)
+        g64.step_rk3(dt)
+        if k % 20 == 19:
+            a, b = g32.diagnostics(0.0), g64.diagnostics(0.0)
+            worst = max(worst, abs(a.dissipation - b.dissipation) / b.dissipation)
+    print("fp32 worst rel eps", worst, flush=True)
+    assert worst < 1e-5
+
+
+def test_graph_and_direct_launch_agree_bitwise():
+    a = sph.TaylorGreenSolver(32, 1600.0, "fp64")
+    b = sph.TaylorGreenSolver(32, 1600.0, "fp64")
+    b.set_graph(False)
+    for _ in range(3):
+        a.step_rk3(1e-3)
+        b.step_rk3(1e-3)
+    for c in range(3):
+        assert np.array_equal(a.u_hat(c), b.u_hat(c))
+
+
+def test_time_passes_leaves_state_unchanged():
+    s = sph.TaylorGreenSolver(32, 1600.0, "fp32")
+    s.step_rk3(1e-3)
+    before = [s.u_hat(c) for c in range(3)]
+    t = s.time_passes(2)
+    assert all(v[0] > 0 for v in t.values())
+    for c in range(3):
+        assert np.array_equal(before[c], s.u_hat(c))
+
+
+def test_reference_cpp_tests_binary():
+    exe = os.path.join(ROOT, "spectral_solver_amd", "spectral_ns_tests")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "all C++ spectral tests passed" in r.stdout
+
+
+def test_cli_fixed_dt(tmp_path):
+    exe = os.path.join(ROOT, "spectral_solver_amd", "taylor_green")
+    r = subprocess.run([exe, "--n", "32", "--steps", "4", "--dt", "1e-3", "--output-dir", str(tmp_path),
+                        "--precision", "fp32"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    rows = open(os.path.join(str(tmp_path), "taylor_green_diagnostics.csv")).read().strip().splitlines()
+    assert len(rows) == 6 and rows[0].startswith("t,kinetic_energy,dissipation")
+    assert "profile_solver" in r.stdout
+    r = subprocess.run([exe, "--n", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "error: grid size must be positive" in r.stderr
