@@ -226,6 +226,162 @@ __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-direct tile transform for strided (z) passes.
+//
+// NF fields of NL lines each. Thread tid owns line l = tid % NL of every field;
+// in the first stage it holds elements j + r*N/R0 (j = tid/NL + i*NT/NL), which
+// the caller loads straight from global memory (for fixed r a wavefront reads
+// NL consecutive values of NT/NL/... rows: whole 128-byte segments). The first
+// stage runs in registers, the middle stages go through LDS, and the last
+// stage leaves element j + r*N/RL of line l in registers for the caller's
+// post-operation and coalesced store.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+struct ZTile {
+    static constexpr RadixPlan P = radix_plan(N);
+    static constexpr int lcm_radix() {
+        int l = 1;
+        for (int s = 0; s < P.count; ++s) {
+            int a = l, b = P.r[s];
+            while (b) {
+                const int t = a % b;
+                a = b;
+                b = t;
+            }
+            l = l / a * P.r[s];
+        }
+        return l;
+    }
+    static constexpr int pick_nl() {
+        // two field tiles must fit in LDS: FP64 at N = 1024 takes narrow tiles
+        int nl = sizeof(T) == 4 ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 2 : 8);
+        while (nl * N / lcm_radix() < 64) nl *= 2;
+        return nl;
+    }
+    static constexpr int NL = pick_nl();
+    static constexpr int E = lcm_radix();  // elements per thread and field
+    static constexpr int NT = NL * N / E;
+    static constexpr int S = NL + 1;           // LDS row stride
+    static constexpr int LDS = S * N;          // complex values per field tile
+    static constexpr int R0 = P.r[0];
+    static constexpr int RL = P.r[P.count - 1];
+    static constexpr int T0 = E / R0;          // first-stage tasks per thread
+    static constexpr int TLAST = E / RL;       // last-stage tasks per thread
+    static constexpr int STEP = NT / NL;       // j step between a thread's tasks
+    static_assert(NT <= 1024 && NT % 64 == 0 && NT % NL == 0, "bad z tile");
+    static_assert(E % R0 == 0 && E % RL == 0, "bad z tile");
+    // element index of first-stage slot (i, r) / last-stage slot (i, r)
+    __device__ static int e_first(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / R0); }
+    __device__ static int e_last(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / RL); }
+};
+
+// One in-place middle Stockham stage over NF field tiles (read all, sync,
+// write all, sync).
+template <typename T, int N, int NL, int NT, int NF, int R, int NS, bool INV>
+__device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid) {
+    constexpr int M = N / R;
+    constexpr int TASKS = NL * M;
+    static_assert(TASKS % NT == 0, "middle stage must split evenly");
+    constexpr int TPT = TASKS / NT;
+    constexpr int S = NL + 1;
+    cx<T> v[NF][TPT][R];
+#pragma unroll
+    for (int i = 0; i < TPT; ++i) {
+        const int t = tid + i * NT;
+        const int line = t % NL, j = t / NL, k = j % NS;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[f][i][r] = lds[f * (S * N) + (j + r * M) * S + line];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cx<T> w = twl[r * k * (N / (NS * R))];
+                if constexpr (INV) w.y = -w.y;
+                v[f][i][r] = cmul(v[f][i][r], w);
+            }
+            dft<R, INV>(v[f][i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TPT; ++i) {
+        const int t = tid + i * NT;
+        const int line = t % NL, j = t / NL, k = j % NS;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[f * (S * N) + (base + r * NS) * S + line] = v[f][i][r];
+    }
+    __syncthreads();
+}
+
+// stages S .. count-2; NS is the product of the radices before stage S
+template <typename T, int N, int NL, int NT, int NF, bool INV, int S, int NS>
+__device__ __forceinline__ void zstages_mid(cx<T>* lds, const cx<T>* twl, int tid) {
+    constexpr RadixPlan P = radix_plan(N);
+    if constexpr (S <= P.count - 2) {
+        zstage_mid<T, N, NL, NT, NF, P.r[S], NS, INV>(lds, twl, tid);
+        zstages_mid<T, N, NL, NT, NF, INV, S + 1, NS * P.r[S]>(lds, twl, tid);
+    }
+}
+
+// v: first-stage inputs (consumed), w: last-stage outputs.
+template <typename T, int N, int NF, bool INV, int A0, int B0, int A1, int B1>
+__device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
+                                     int tid) {
+    using Z = ZTile<T, N>;
+    static_assert(A0 == Z::T0 && B0 == Z::R0 && A1 == Z::TLAST && B1 == Z::RL, "slot arrays must match the tile");
+    constexpr int NL = Z::NL, NT = Z::NT, S = Z::S, R0 = Z::R0, RL = Z::RL;
+    constexpr RadixPlan P = Z::P;
+    if constexpr (P.count == 1) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < Z::T0; ++i) {
+                dft<R0, INV>(v[f][i]);
+#pragma unroll
+                for (int r = 0; r < R0; ++r) w[f][i][r] = v[f][i][r];
+            }
+    } else {
+        const int l = tid % NL;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < Z::T0; ++i) dft<R0, INV>(v[f][i]);
+        __syncthreads();  // previous readers of the tile buffers are done
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < Z::T0; ++i) {
+                const int j = tid / NL + i * Z::STEP;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) lds[f * Z::LDS + (j * R0 + r) * S + l] = v[f][i][r];
+            }
+        __syncthreads();
+        // stages 1 .. count-2 (NS after stage 0 is R0)
+        if constexpr (P.count > 2) zstages_mid<T, N, NL, NT, NF, INV, 1, R0>(lds, twl, tid);
+        constexpr int NSL = N / RL;
+#pragma unroll
+        for (int i = 0; i < Z::TLAST; ++i) {
+            const int j = tid / NL + i * Z::STEP;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) w[f][i][r] = lds[f * Z::LDS + (j + r * NSL) * S + l];
+#pragma unroll
+                for (int r = 1; r < RL; ++r) {
+                    cx<T> tw = twl[r * j];
+                    if constexpr (INV) tw.y = -tw.y;
+                    w[f][i][r] = cmul(w[f][i][r], tw);
+                }
+                dft<RL, INV>(w[f][i]);
+            }
+        }
+    }
+}
+
 template <typename T, int N, int NT>
 __device__ __forceinline__ void load_twiddles(cx<T>* twl, const cx<T>* __restrict__ tw, int tid) {
 #pragma unroll

@@ -29,14 +29,15 @@ const PassTable<T>* pass_table(int n);
 template <typename T, int N>
 struct Launch {
     using TL = Tile<T, N>;
+    using ZT = ZTile<T, N>;
     static void inv_z(const InvZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
-        const dim3 grid((nky + TL::NL - 1) / TL::NL, nkx);
+        const dim3 grid((nky + ZT::NL - 1) / ZT::NL, nkx);
         if (mode == IZ_RHS)
-            k_inv_z<T, N, IZ_RHS><<<grid, TL::NT, 0, s>>>(a);
+            k_inv_z<T, N, IZ_RHS><<<grid, ZT::NT, 0, s>>>(a);
         else if (mode == IZ_DIAG)
-            k_inv_z<T, N, IZ_DIAG><<<grid, TL::NT, 0, s>>>(a);
+            k_inv_z<T, N, IZ_DIAG><<<grid, ZT::NT, 0, s>>>(a);
         else
-            k_inv_z<T, N, IZ_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+            k_inv_z<T, N, IZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
     }
     static void lines(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t s) {
         const dim3 grid(static_cast<unsigned>((nlines + TL::NL - 1) / TL::NL));
@@ -57,13 +58,13 @@ struct Launch {
             k_fwd_x<T, N, FX_PLAIN><<<grid, TL::NT, 0, s>>>(a);
     }
     static void fwd_z(const FwdZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
-        const dim3 grid((nky + TL::NL - 1) / TL::NL, nkx);
+        const dim3 grid((nky + ZT::NL - 1) / ZT::NL, nkx);
         if (mode == FZ_RK)
-            k_fwd_z<T, N, FZ_RK><<<grid, TL::NT, 0, s>>>(a);
+            k_fwd_z<T, N, FZ_RK><<<grid, ZT::NT, 0, s>>>(a);
         else if (mode == FZ_PROJECT)
-            k_fwd_z<T, N, FZ_PROJECT><<<grid, TL::NT, 0, s>>>(a);
+            k_fwd_z<T, N, FZ_PROJECT><<<grid, ZT::NT, 0, s>>>(a);
         else
-            k_fwd_z<T, N, FZ_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+            k_fwd_z<T, N, FZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
     }
     static constexpr PassTable<T> table{N, TL::NL, TL::NT, &inv_z, &lines, &inv_x, &fwd_x, &fwd_z};
 };

@@ -48,6 +48,44 @@ __device__ __forceinline__ bool keep_mode(int k2, int n) {
     return 4LL * k2 <= static_cast<long long>(n) * n;
 }
 
+// Raw buffer access: a work-group-uniform resource (scalar registers) plus a
+// 32-bit byte offset per lane, so one offset register per slot serves every
+// array of the same layout. Accesses beyond `bytes` read 0 / are dropped.
+// Resource word 3 for gfx9-family raw buffers: 0x00020000.
+template <typename T>
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const cx<T>* base, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<cx<T>*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
+    __device__ __forceinline__ cx<T> ld(uint32_t off) const {
+        if constexpr (sizeof(T) == 4) {
+            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+        } else {
+            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        }
+    }
+    __device__ __forceinline__ void st(uint32_t off, cx<T> v) const {
+        if constexpr (sizeof(T) == 4) {
+            using V = int __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
+        } else {
+            using V = int __attribute__((ext_vector_type(4)));
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, off, 0, 0);
+        }
+    }
+};
+
+// Pointer q of an argument array, selected with a uniform index: a chain of
+// scalar selects, so the array is never copied to private memory (dynamic
+// indexing of a by-value kernel argument would be).
+template <typename P, int K>
+__device__ __forceinline__ P pick(P const (&arr)[K], int q) {
+    P r = arr[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) r = q == k ? arr[k] : r;
+    return r;
+}
+
 template <typename T>
 struct InvZArgs {
     const cx<T>* in[3];
@@ -99,119 +137,93 @@ struct FwdZArgs {
 };
 
 // ---------------------------------------------------------------------------
-// inverse z pass
+// inverse z pass (register-direct first/last stages, see zfft)
 // ---------------------------------------------------------------------------
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Tile<T, N>::NT)) k_inv_z(InvZArgs<T> a) {
-    using TL = Tile<T, N>;
-    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
-    __shared__ cx<T> buf[TL::LDS];
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_inv_z(InvZArgs<T> a) {
+    using Z = ZTile<T, N>;
+    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    constexpr int NF = MODE == IZ_PLAIN ? 1 : 2;
+    __shared__ cx<T> lds[NF * Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
-    const int ky0 = blockIdx.x * NL;
+    const int ky = blockIdx.x * NL + tid % NL;
     const int kx = blockIdx.y;
+    const bool ok = ky < a.ny;
+    // loads of lines beyond ny read line ny-1 (in bounds) and are never stored
+    const uint32_t lane = static_cast<uint32_t>((tid / NL) * a.ny + (ok ? ky : a.ny - 1)) * sizeof(cx<T>);
+    const uint32_t rowb = static_cast<uint32_t>(a.ny) * sizeof(cx<T>);
+    const size_t plane = static_cast<size_t>(kx) * N * a.ny;
+    const uint32_t plane_bytes = N * rowb;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
-
-    // line-fast items: thread owns line l and elements e0 + i*STEP
-    static_assert(NT % NL == 0, "tile threads must cover whole rows");
-    constexpr int STEP = NT / NL;
-    const int l = tid % NL, e0 = tid / NL;
-    const bool lane_ok = ky0 + l < a.ny;
-    const int off0 = (kx * N + e0) * a.ny + ky0 + l;
-    const int stride_off = STEP * a.ny;
-    const int li0 = e0 * S + l;
-#define SPH_IN(i) ((TL::ELEMS % NT == 0) || (e0 + (i) * STEP < N))
-#define SPH_OK(i) (lane_ok && SPH_IN(i))
-#define SPH_OFF(i) (off0 + (i) * stride_off)
-#define SPH_LI(i) (SPH_IN(i) ? li0 + (i) * STEP * S : 0)
+    auto load = [&](const cx<T>* src, cx<T> (&v)[T0][R0]) {
+        const Buf<T> b(src + plane, plane_bytes);
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[i][r] = b.ld(lane + (i * Z::STEP + r * (N / R0)) * rowb);
+    };
+    auto store = [&](cx<T>* dst, const cx<T> (&w)[TLS][RL]) {
+        if (!ok) return;
+        const Buf<T> b(dst + plane, plane_bytes);
+#pragma unroll
+        for (int i = 0; i < TLS; ++i)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) b.st(lane + (i * Z::STEP + r * (N / RL)) * rowb, w[i][r]);
+    };
+    cx<T> v[NF][T0][R0];
+    cx<T> w[NF][TLS][RL];
     if constexpr (MODE == IZ_PLAIN) {
         for (int f = 0; f < a.nfields; ++f) {
-#pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_IN(i)) buf[SPH_LI(i)] = SPH_OK(i) ? a.in[f][SPH_OFF(i)] : czero<T>();
-            __syncthreads();
-            fft_tile<T, N, NL, NT, true>(buf, twl, tid);
-#pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[f][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
+            load(pick(a.in, f), v[0]);
+            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            store(pick(a.out, f), w[0]);
         }
     } else if constexpr (MODE == IZ_RHS) {
+        const int my = signed_mode(ky, a.n);
 #pragma unroll 1
         for (int c = 0; c < 3; ++c) {
-            cx<T> u[E];
+            load(pick(a.in, c), v[0]);
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                u[i] = SPH_OK(i) ? a.in[c][SPH_OFF(i)] : czero<T>();
-                if (SPH_IN(i)) buf[SPH_LI(i)] = u[i];
-            }
-            __syncthreads();
-            fft_tile<T, N, NL, NT, true>(buf, twl, tid);
+            for (int i = 0; i < T0; ++i)
 #pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[2 * c][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                const int idx = tid + i * NT;
-                const int my = signed_mode(ky0 + idx % NL, a.n), mz = signed_mode(idx / NL, a.n);
-                const cx<T> ph = SPH_OK(i) ? a.phase[kx + my + mz + a.n] : czero<T>();
-                if (SPH_IN(i)) buf[SPH_LI(i)] = cmul(u[i], ph);
-            }
-            __syncthreads();
-            fft_tile<T, N, NL, NT, true>(buf, twl, tid);
-#pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[2 * c + 1][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
+                for (int r = 0; r < R0; ++r) {
+                    const int mz = signed_mode(Z::e_first(tid, i, r), a.n);
+                    const cx<T> ph = a.phase[kx + my + mz + a.n];
+                    v[1][i][r] = cmul(v[0][i][r], ph);
+                }
+            zfft<T, N, 2, true>(v, w, lds, twl, tid);
+            store(pick(a.out, 2 * c), w[0]);
+            store(pick(a.out, 2 * c + 1), w[1]);
         }
-    } else {  // IZ_DIAG
-        cx<T> u0[E], u1[E], u2[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            u0[i] = SPH_OK(i) ? a.in[0][SPH_OFF(i)] : czero<T>();
-            u1[i] = SPH_OK(i) ? a.in[1][SPH_OFF(i)] : czero<T>();
-            u2[i] = SPH_OK(i) ? a.in[2][SPH_OFF(i)] : czero<T>();
-        }
+    } else {  // IZ_DIAG: u and omega = i k x u
+        cx<T> u[3][T0][R0];
+        load(a.in[0], u[0]);
+        load(a.in[1], u[1]);
+        load(a.in[2], u[2]);
+        const T ix = T(kx), iy = T(signed_mode(ky, a.n));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
 #pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_IN(i)) buf[SPH_LI(i)] = c == 0 ? u0[i] : (c == 1 ? u1[i] : u2[i]);
-            __syncthreads();
-            fft_tile<T, N, NL, NT, true>(buf, twl, tid);
+            for (int i = 0; i < T0; ++i)
 #pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[2 * c][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                const int idx = tid + i * NT;
-                const T ix = T(kx), iy = T(signed_mode(ky0 + idx % NL, a.n)), iz = T(signed_mode(idx / NL, a.n));
-                cx<T> w;
-                if (c == 0) {
-                    w = {-(iy * u2[i].y - iz * u1[i].y), iy * u2[i].x - iz * u1[i].x};
-                } else if (c == 1) {
-                    w = {-(iz * u0[i].y - ix * u2[i].y), iz * u0[i].x - ix * u2[i].x};
-                } else {
-                    w = {-(ix * u1[i].y - iy * u0[i].y), ix * u1[i].x - iy * u0[i].x};
+                for (int r = 0; r < R0; ++r) {
+                    const T iz = T(signed_mode(Z::e_first(tid, i, r), a.n));
+                    const cx<T> u0 = u[0][i][r], u1 = u[1][i][r], u2 = u[2][i][r];
+                    v[0][i][r] = u[c][i][r];
+                    if (c == 0)
+                        v[1][i][r] = {-(iy * u2.y - iz * u1.y), iy * u2.x - iz * u1.x};
+                    else if (c == 1)
+                        v[1][i][r] = {-(iz * u0.y - ix * u2.y), iz * u0.x - ix * u2.x};
+                    else
+                        v[1][i][r] = {-(ix * u1.y - iy * u0.y), ix * u1.x - iy * u0.x};
                 }
-                if (SPH_IN(i)) buf[SPH_LI(i)] = w;
-            }
-            __syncthreads();
-            fft_tile<T, N, NL, NT, true>(buf, twl, tid);
-#pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[2 * c + 1][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
+            zfft<T, N, 2, true>(v, w, lds, twl, tid);
+            store(a.out[2 * c], w[0]);
+            store(a.out[2 * c + 1], w[1]);
         }
     }
 }
-
-#undef SPH_IN
-#undef SPH_OK
-#undef SPH_OFF
-#undef SPH_LI
 
 // ---------------------------------------------------------------------------
 // contiguous lines (y pass), in place
@@ -261,8 +273,8 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_inv_x(InvXArgs<T> a) {
     load_twiddles<T, N, NT>(twl, a.tw, tid);
 #pragma unroll 1
     for (int p = 0; p < a.npairs; ++p) {
-        const cx<T>* A = a.a[p];
-        const cx<T>* B = a.b[p];
+        const cx<T>* A = pick(a.a, p);
+        const cx<T>* B = pick(a.b, p);
 #pragma unroll
         for (int i = 0; i < LPT; ++i) {
             const int idx = tid + i * NT;
@@ -284,8 +296,8 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_inv_x(InvXArgs<T> a) {
         }
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
-        T* OA = a.oa[p];
-        T* OB = a.ob[p];
+        T* OA = pick(a.oa, p);
+        T* OB = pick(a.ob, p);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int idx = tid + i * NT;
@@ -376,16 +388,16 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_fwd_x(FwdXArgs<T> a) {
     } else {  // FX_PLAIN
 #pragma unroll 1
         for (int p = 0; 2 * p < a.nin; ++p) {
-            const T* fa = a.in[2 * p];
-            const T* fb = 2 * p + 1 < a.nin ? a.in[2 * p + 1] : nullptr;
+            const T* fa = pick(a.in, 2 * p);
+            const T* fb = 2 * p + 1 < a.nin ? pick(a.in, 2 * p + 1) : nullptr;
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 if (in[i]) buf[li[i]] = {ok[i] ? fa[off[i]] : T(0), (ok[i] && fb) ? fb[off[i]] : T(0)};
             }
             __syncthreads();
             fft_tile<T, N, NL, NT, false>(buf, twl, tid);
-            fwd_x_separate_store<T, N>(buf, a.out[2 * p], fb ? a.out[2 * p + 1] : nullptr, y0, z, a.ny, a.nz,
-                                       tid);
+            fwd_x_separate_store<T, N>(buf, pick(a.out, 2 * p), fb ? pick(a.out, 2 * p + 1) : nullptr, y0, z, a.ny,
+                                       a.nz, tid);
             __syncthreads();
         }
     }
@@ -394,183 +406,184 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_fwd_x(FwdXArgs<T> a) {
 // ---------------------------------------------------------------------------
 // forward z pass (+ fused RHS completion and RK3 update)
 // ---------------------------------------------------------------------------
+// Leray projection of f (impl.hpp:829-843): f -= k (k.f) * (1/k2) for kept
+// modes with k2 > 0, zero otherwise; evaluated without branches.
 template <typename T>
 __device__ __forceinline__ void project_or_zero(cx<T>& f0, cx<T>& f1, cx<T>& f2, int ix, int iy, int iz, int k2,
                                                 bool keep) {
-    const T rk2 = T(k2);
-    if (k2 > 0 && keep) {
-        const T fx = T(ix), fy = T(iy), fz = T(iz);
-        const cx<T> dot = (f0 * fx + f1 * fy) + f2 * fz;
-        const cx<T> fac = dot * (T(1) / rk2);
-        f0 = f0 - fac * fx;
-        f1 = f1 - fac * fy;
-        f2 = f2 - fac * fz;
-    } else {
-        f0 = f1 = f2 = czero<T>();
-    }
+    const bool live = k2 > 0 && keep;
+    const T inv = T(1) / T(live ? k2 : 1);
+    const T fx = T(ix), fy = T(iy), fz = T(iz);
+    const cx<T> dot = (f0 * fx + f1 * fy) + f2 * fz;
+    const cx<T> fac = dot * inv;
+    const cx<T> g0 = f0 - fac * fx, g1 = f1 - fac * fy, g2 = f2 - fac * fz;
+    f0 = live ? g0 : czero<T>();
+    f1 = live ? g1 : czero<T>();
+    f2 = live ? g2 : czero<T>();
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((Tile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
-    using TL = Tile<T, N>;
-    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
-    __shared__ cx<T> buf[TL::LDS];
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
+    using Z = ZTile<T, N>;
+    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    constexpr int NF = MODE == FZ_RK ? 2 : 1;
+    __shared__ cx<T> lds[NF * Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
-    const int ky0 = blockIdx.x * NL;
+    const int ky = blockIdx.x * NL + tid % NL;
     const int kx = blockIdx.y;
+    const bool ok = ky < a.ny;
+    const uint32_t lane = static_cast<uint32_t>((tid / NL) * a.ny + (ok ? ky : a.ny - 1)) * sizeof(cx<T>);
+    const uint32_t rowb = static_cast<uint32_t>(a.ny) * sizeof(cx<T>);
+    const size_t plane = static_cast<size_t>(kx) * N * a.ny;
+    const uint32_t plane_bytes = N * rowb;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
-
-    // line-fast items: thread owns line l and elements e0 + i*STEP
-    static_assert(NT % NL == 0, "tile threads must cover whole rows");
-    constexpr int STEP = NT / NL;
-    const int l = tid % NL, e0 = tid / NL;
-    const bool lane_ok = ky0 + l < a.ny;
-    const int off0 = (kx * N + e0) * a.ny + ky0 + l;
-    const int stride_off = STEP * a.ny;
-    const int li0 = e0 * S + l;
-#define SPH_IN(i) ((TL::ELEMS % NT == 0) || (e0 + (i) * STEP < N))
-#define SPH_OK(i) (lane_ok && SPH_IN(i))
-#define SPH_OFF(i) (off0 + (i) * stride_off)
-#define SPH_LI(i) (SPH_IN(i) ? li0 + (i) * STEP * S : 0)
-    auto transform = [&](const cx<T>* src) {
+    auto load = [&](const cx<T>* src, cx<T> (&v)[T0][R0]) {
+        const Buf<T> b(src + plane, plane_bytes);
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            if (SPH_IN(i)) buf[SPH_LI(i)] = SPH_OK(i) ? src[SPH_OFF(i)] : czero<T>();
-        }
-        __syncthreads();
-        fft_tile<T, N, NL, NT, false>(buf, twl, tid);
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[i][r] = b.ld(lane + (i * Z::STEP + r * (N / R0)) * rowb);
     };
-
+    auto last_off = [&](int i, int r) -> uint32_t { return lane + (i * Z::STEP + r * (N / RL)) * rowb; };
+    cx<T> v[NF][T0][R0];
+    cx<T> w[NF][TLS][RL];
     if constexpr (MODE == FZ_PLAIN) {
-#pragma unroll 1
         for (int f = 0; f < a.nfields; ++f) {
-            transform(a.in[f]);
+            load(pick(a.in, f), v[0]);
+            zfft<T, N, 1, false>(v, w, lds, twl, tid);
+            if (ok) {
+                const Buf<T> o(pick(a.out, f) + plane, plane_bytes);
 #pragma unroll
-            for (int i = 0; i < E; ++i)
-                if (SPH_OK(i)) a.out[f][SPH_OFF(i)] = buf[SPH_LI(i)];
-            __syncthreads();
+                for (int i = 0; i < TLS; ++i)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) o.st(last_off(i, r), w[0][i][r]);
+            }
         }
     } else if constexpr (MODE == FZ_PROJECT) {
-        cx<T> v[3][E];
+        cx<T> u[3][TLS][RL];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            transform(a.in[c]);
+            load(a.in[c], v[0]);
+            zfft<T, N, 1, false>(v, w, lds, twl, tid);
 #pragma unroll
-            for (int i = 0; i < E; ++i) v[c][i] = buf[SPH_LI(i)];
-            __syncthreads();
+            for (int i = 0; i < TLS; ++i)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) u[c][i][r] = w[0][i][r];
         }
+        if (ok) {
+            const int my = signed_mode(ky, a.n);
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            if (!SPH_OK(i)) continue;
-            const int idx = tid + i * NT;
-            const int ix = kx, iy = signed_mode(ky0 + idx % NL, a.n), iz = signed_mode(idx / NL, a.n);
-            const int k2 = ix * ix + iy * iy + iz * iz;
-            project_or_zero(v[0][i], v[1][i], v[2][i], ix, iy, iz, k2, keep_mode(k2, a.n));
+            for (int i = 0; i < TLS; ++i)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) a.out[c][SPH_OFF(i)] = v[c][i];
+                for (int r = 0; r < RL; ++r) {
+                    const int e = Z::e_last(tid, i, r);
+                    const int mz = signed_mode(e, a.n);
+                    const int k2 = kx * kx + my * my + mz * mz;
+                    project_or_zero(u[0][i][r], u[1][i][r], u[2][i][r], kx, my, mz, k2, keep_mode(k2, a.n));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) Buf<T>(a.out[c] + plane, plane_bytes).st(last_off(i, r), u[c][i][r]);
+                }
         }
     } else {  // FZ_RK
-        // Every thread owns the same line l = tid % NL for all of its items
-        // (NT is a multiple of NL), so my is per-thread and mz per item.
-        static_assert(NT % NL == 0, "tile threads must cover whole rows");
-        const int my = signed_mode(ky0 + l, a.n);
+        const int my = signed_mode(ky, a.n);
         const T fx = T(kx), fy = T(my);
-        cx<T> r0[E], r1[E], r2[E];
+        cx<T> r0[TLS][RL], r1[TLS][RL], r2[TLS][RL];
 #pragma unroll
-        for (int i = 0; i < E; ++i) r0[i] = r1[i] = r2[i] = czero<T>();
-        // When two tile buffers fit in LDS, the plain product's transform is
-        // parked in the second while the shifted product is transformed in the
-        // first; otherwise it is held in registers.
-        constexpr bool kPark = (2 * TL::LDS + N) * sizeof(cx<T>) <= 160 * 1024;
-        __shared__ cx<T> bufp[kPark ? TL::LDS : 1];
-        cx<T> Preg[kPark ? 1 : E];
+        for (int i = 0; i < TLS; ++i)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) r0[i][r] = r1[i][r] = r2[i][r] = czero<T>();
         // products in the reference's accumulation order (00,01,02,11,12,22)
 #pragma unroll 1
         for (int p = 0; p < 6; ++p) {
             const int pa = p < 3 ? 0 : (p < 5 ? 1 : 2);
             const int pb = p < 3 ? p : (p < 5 ? p - 2 : 2);
-            if constexpr (kPark) {
+#if SPH_RK_PAIRED || !defined(SPH_RK_PAIRED)
+            load(pick(a.in, 2 * p), v[0]);
+            load(pick(a.in, 2 * p + 1), v[1]);
+            zfft<T, N, 2, false>(v, w, lds, twl, tid);
+#else
+            {
+                cx<T> v1[1][T0][R0], w1[1][TLS][RL];
+                load(pick(a.in, 2 * p), v1[0]);
+                zfft<T, N, 1, false>(v1, w1, lds, twl, tid);
 #pragma unroll
-                for (int i = 0; i < E; ++i)
-                    if (SPH_IN(i)) bufp[SPH_LI(i)] = SPH_OK(i) ? a.in[2 * p][SPH_OFF(i)] : czero<T>();
-                __syncthreads();
-                fft_tile<T, N, NL, NT, false>(bufp, twl, tid);
-            } else {
-                transform(a.in[2 * p]);
+                for (int i = 0; i < TLS; ++i)
 #pragma unroll
-                for (int i = 0; i < E; ++i) Preg[i] = buf[SPH_LI(i)];
-                __syncthreads();
+                    for (int r = 0; r < RL; ++r) w[0][i][r] = w1[0][i][r];
+                load(pick(a.in, 2 * p + 1), v1[0]);
+                zfft<T, N, 1, false>(v1, w1, lds, twl, tid);
+#pragma unroll
+                for (int i = 0; i < TLS; ++i)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) w[1][i][r] = w1[0][i][r];
             }
-            transform(a.in[2 * p + 1]);
+#endif
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                const int mz = signed_mode(e0 + i * STEP, a.n);
-                const int k2 = kx * kx + my * my + mz * mz;
-                const cx<T> Q = buf[SPH_LI(i)];
-                cx<T> Pv;
-                if constexpr (kPark)
-                    Pv = bufp[SPH_LI(i)];
-                else
-                    Pv = Preg[i];
-                const cx<T> ph = SPH_OK(i) ? a.phase[kx + my + mz + a.n] : czero<T>();
-                cx<T> v = (Pv + cmul(Q, cx<T>{ph.x, -ph.y})) * T(0.5);
-                if (!keep_mode(k2, a.n)) v = czero<T>();
-                const T fz = T(mz);
-                const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
-                const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
-                const cx<T> da = {kb * v.y, -kb * v.x};  // -i k_b v into r_a
-                const cx<T> db = {ka * v.y, -ka * v.x};  // -i k_a v into r_b
-                if (pa == 0)
-                    r0[i] = r0[i] + da;
-                else if (pa == 1)
-                    r1[i] = r1[i] + da;
-                else
-                    r2[i] = r2[i] + da;
-                if (pa != pb) {
-                    if (pb == 1)
-                        r1[i] = r1[i] + db;
+            for (int i = 0; i < TLS; ++i)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int mz = signed_mode(Z::e_last(tid, i, r), a.n);
+                    const int k2 = kx * kx + my * my + mz * mz;
+                    const cx<T> ph = a.phase[kx + my + mz + a.n];
+                    cx<T> val = (w[0][i][r] + cmul(w[1][i][r], cx<T>{ph.x, -ph.y})) * T(0.5);
+                    if (!keep_mode(k2, a.n)) val = czero<T>();
+                    const T fz = T(mz);
+                    const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
+                    const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
+                    const cx<T> da = {kb * val.y, -kb * val.x};  // -i k_b v into r_a
+                    const cx<T> db = {ka * val.y, -ka * val.x};  // -i k_a v into r_b
+                    if (pa == 0)
+                        r0[i][r] = r0[i][r] + da;
+                    else if (pa == 1)
+                        r1[i][r] = r1[i][r] + da;
                     else
-                        r2[i] = r2[i] + db;
+                        r2[i][r] = r2[i][r] + da;
+                    if (pa != pb) {
+                        if (pb == 1)
+                            r1[i][r] = r1[i][r] + db;
+                        else
+                            r2[i][r] = r2[i][r] + db;
+                    }
                 }
-            }
-            __syncthreads();
         }
+        if (!ok) return;
         const T alpha = a.coef[2 * a.stage];
         const T beta = a.coef[2 * a.stage + 1];
         const bool first = a.stage == 0;
+        const Buf<T> U[3] = {Buf<T>(a.u[0] + plane, plane_bytes), Buf<T>(a.u[1] + plane, plane_bytes),
+                             Buf<T>(a.u[2] + plane, plane_bytes)};
+        const Buf<T> G[3] = {Buf<T>(a.g[0] + plane, plane_bytes), Buf<T>(a.g[1] + plane, plane_bytes),
+                             Buf<T>(a.g[2] + plane, plane_bytes)};
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            if (!SPH_OK(i)) continue;
-            const int mz = signed_mode(e0 + i * STEP, a.n);
-            const int k2 = kx * kx + my * my + mz * mz;
-            const bool keep = keep_mode(k2, a.n);
-            cx<T> u[3] = {a.u[0][SPH_OFF(i)], a.u[1][SPH_OFF(i)], a.u[2][SPH_OFF(i)]};
-            cx<T> r[3] = {r0[i], r1[i], r2[i]};
-            const T vk2 = a.nu * T(k2);
+        for (int i = 0; i < TLS; ++i)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] = r[c] - u[c] * vk2;
-            project_or_zero(r[0], r[1], r[2], kx, my, mz, k2, keep);
-            cx<T> g[3];
+            for (int r = 0; r < RL; ++r) {
+                const uint32_t off = last_off(i, r);
+                const int mz = signed_mode(Z::e_last(tid, i, r), a.n);
+                const int k2 = kx * kx + my * my + mz * mz;
+                const bool keep = keep_mode(k2, a.n);
+                cx<T> u[3] = {U[0].ld(off), U[1].ld(off), U[2].ld(off)};
+                cx<T> rr[3] = {r0[i][r], r1[i][r], r2[i][r]};
+                const T vk2 = a.nu * T(k2);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                g[c] = first ? r[c] : a.g[c][SPH_OFF(i)] * alpha + r[c];
-                u[c] = u[c] + g[c] * beta;
+                for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[c] * vk2;
+                project_or_zero(rr[0], rr[1], rr[2], kx, my, mz, k2, keep);
+                cx<T> g[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    g[c] = first ? rr[c] : G[c].ld(off) * alpha + rr[c];
+                    u[c] = u[c] + g[c] * beta;
+                }
+                project_or_zero(u[0], u[1], u[2], kx, my, mz, k2, keep);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    G[c].st(off, g[c]);
+                    U[c].st(off, u[c]);
+                }
             }
-            project_or_zero(u[0], u[1], u[2], kx, my, mz, k2, keep);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a.g[c][SPH_OFF(i)] = g[c];
-                a.u[c][SPH_OFF(i)] = u[c];
-            }
-        }
     }
 }
-
-#undef SPH_IN
-#undef SPH_OK
-#undef SPH_OFF
-#undef SPH_LI
 
 // ---------------------------------------------------------------------------
 // reductions (FP64 partials)
