@@ -368,13 +368,12 @@ public:
     sph_diagnostics diagnostics(double t) override {
         const auto t0 = std::chrono::steady_clock::now();
         const hipStream_t s = stream_.get();
-        inverse(IZ_DIAG);
+        inverse(PZ_DIAG);
         k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P_.get(), P_.get() + npts_, P_.get() + 2 * npts_,
                                                                    P_.get() + 3 * npts_, P_.get() + 4 * npts_,
                                                                    P_.get() + 5 * npts_, npts_, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
-        k_reduce_div<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(U_.get(), U_.get() + modes, U_.get() + 2 * modes,
-                                                                  n, partial_.get());
+        k_reduce_div3<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(U_.get(), n, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 1, 0b1, result_.get() + 3);
         double r[4];
         hip_check(hipMemcpyAsync(r, result_.get(), sizeof(r), hipMemcpyDeviceToHost, s), "hipMemcpy diagnostics");
@@ -396,7 +395,7 @@ public:
     double max_velocity() override {
         const auto t0 = std::chrono::steady_clock::now();
         const hipStream_t s = stream_.get();
-        inverse(IZ_PLAIN);
+        inverse(PZ_VEL);
         k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P_.get(), P_.get() + npts_, P_.get() + 2 * npts_,
                                                                    nullptr, nullptr, nullptr, npts_, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
@@ -412,8 +411,9 @@ public:
     void get_uhat(int c, double* host) override {
         if (c < 0 || c > 2) throw std::runtime_error("component out of range");
         std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
-        hip_check(hipMemcpyAsync(tmp.data(), U_.get() + c * modes, modes * sizeof(cx<T>), hipMemcpyDeviceToHost,
-                                 stream_.get()),
+        // component c of the component-interleaved state: a strided 2D copy
+        hip_check(hipMemcpy2DAsync(tmp.data(), sizeof(cx<T>), U_.get() + c, 3 * sizeof(cx<T>), sizeof(cx<T>), modes,
+                                   hipMemcpyDeviceToHost, stream_.get()),
                   "hipMemcpy get_uhat");
         hip_check(hipStreamSynchronize(stream_.get()), "get_uhat");
         for (std::int64_t i = 0; i < modes; ++i) {
@@ -426,8 +426,8 @@ public:
         if (c < 0 || c > 2) throw std::runtime_error("component out of range");
         std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
         for (std::int64_t i = 0; i < modes; ++i) tmp[i] = {static_cast<T>(host[2 * i]), static_cast<T>(host[2 * i + 1])};
-        hip_check(hipMemcpyAsync(U_.get() + c * modes, tmp.data(), modes * sizeof(cx<T>), hipMemcpyHostToDevice,
-                                 stream_.get()),
+        hip_check(hipMemcpy2DAsync(U_.get() + c, 3 * sizeof(cx<T>), tmp.data(), sizeof(cx<T>), sizeof(cx<T>), modes,
+                                   hipMemcpyHostToDevice, stream_.get()),
                   "hipMemcpy set_uhat");
         hip_check(hipStreamSynchronize(stream_.get()), "set_uhat");
     }
@@ -500,29 +500,30 @@ private:
     }
 
     // pass k of stage `stage` (0 inv_z, 1 inv_y, 2 inv_x, 3 fwd_x, 4 fwd_y, 5 fwd_z+update)
+    // on the pair-interleaved work buffer W (pair field p at W + 2*p*modes) and
+    // the component-interleaved state u, g
     void launch_pass(int k, int stage, cx<T>* u, cx<T>* g) {
         const hipStream_t s = stream_.get();
-        const std::int64_t M = modes;
+        const std::int64_t M2 = 2 * modes;
         switch (k) {
             case 0: {
-                InvZArgs<T> a{};
-                for (int c = 0; c < 3; ++c) a.in[c] = u + c * M;
-                for (int f = 0; f < 6; ++f) a.out[f] = W_.get() + f * M;
+                PairInvZArgs<T> a{};
+                a.u3 = u;
+                for (int c = 0; c < 3; ++c) a.out[c] = W_.get() + c * M2;
                 a.ny = n;
                 a.n = n;
                 a.phase = phase_.get();
                 a.tw = tw_.get();
-                pt_->inv_z(a, IZ_RHS, n, H_, s);
+                pt_->pinv_z(a, PZ_RHS, n, H_, s);
                 break;
             }
             case 1:
-                pt_->lines(W_.get(), 6LL * H_ * n, true, tw_.get(), s);
+                pt_->plines(W_.get(), 3LL * H_ * n, true, tw_.get(), s);
                 break;
             case 2: {
-                InvXArgs<T> a{};
+                PairInvXArgs<T> a{};
                 for (int c = 0; c < 3; ++c) {
-                    a.a[c] = W_.get() + (2 * c) * M;
-                    a.b[c] = W_.get() + (2 * c + 1) * M;
+                    a.ab[c] = W_.get() + c * M2;
                     a.oa[c] = P_.get() + c * npts_;
                     a.ob[c] = P_.get() + (3 + c) * npts_;
                 }
@@ -531,29 +532,27 @@ private:
                 a.nz = n;
                 a.scale = T(1) / static_cast<T>(npts_);
                 a.tw = tw_.get();
-                pt_->inv_x(a, n, n, s);
+                pt_->pinv_x(a, n, n, s);
                 break;
             }
             case 3: {
-                FwdXArgs<T> a{};
+                PairFwdXArgs<T> a{};
                 for (int f = 0; f < 6; ++f) a.in[f] = P_.get() + f * npts_;
-                for (int q = 0; q < 12; ++q) a.out[q] = W_.get() + q * M;
+                for (int q = 0; q < 6; ++q) a.out[q] = W_.get() + q * M2;
                 a.ny = n;
                 a.nz = n;
                 a.tw = tw_.get();
-                pt_->fwd_x(a, FX_PRODUCTS, n, n, s);
+                pt_->pfwd_x(a, n, n, s);
                 break;
             }
             case 4:
-                pt_->lines(W_.get(), 12LL * H_ * n, false, tw_.get(), s);
+                pt_->plines(W_.get(), 6LL * H_ * n, false, tw_.get(), s);
                 break;
             default: {
-                FwdZArgs<T> a{};
-                for (int q = 0; q < 12; ++q) a.in[q] = W_.get() + q * M;
-                for (int c = 0; c < 3; ++c) {
-                    a.u[c] = u + c * M;
-                    a.g[c] = g + c * M;
-                }
+                PairFwdZArgs<T> a{};
+                for (int q = 0; q < 6; ++q) a.in[q] = W_.get() + q * M2;
+                a.u3 = u;
+                a.g3 = g;
                 a.ny = n;
                 a.n = n;
                 a.nu = nu_;
@@ -561,7 +560,7 @@ private:
                 a.stage = stage;
                 a.phase = phase_.get();
                 a.tw = tw_.get();
-                pt_->fwd_z(a, FZ_RK, n, H_, s);
+                pt_->pfwd_z_rk(a, n, H_, s);
                 break;
             }
         }
@@ -571,49 +570,44 @@ private:
         for (int k = 0; k < 6; ++k) launch_pass(k, stage, U_.get(), G_.get());
     }
 
-    // u_hat -> physical: IZ_DIAG gives u in P[0..2] and omega in P[3..5];
-    // IZ_PLAIN gives u in P[0..2] only.
+    // u_hat -> physical: PZ_DIAG gives u in P[0..2] and omega in P[3..5];
+    // PZ_VEL gives u in P[0..2] only.
     void inverse(int mode) {
         const hipStream_t s = stream_.get();
-        const std::int64_t M = modes;
-        InvZArgs<T> a{};
-        for (int c = 0; c < 3; ++c) a.in[c] = U_.get() + c * M;
+        const std::int64_t M2 = 2 * modes;
+        PairInvZArgs<T> a{};
+        a.u3 = U_.get();
+        for (int c = 0; c < 3; ++c) a.out[c] = W_.get() + c * M2;
         a.ny = n;
         a.n = n;
         a.phase = phase_.get();
         a.tw = tw_.get();
-        InvXArgs<T> x{};
+        PairInvXArgs<T> x{};
         x.ny = n;
         x.nz = n;
         x.scale = T(1) / static_cast<T>(npts_);
         x.tw = tw_.get();
-        if (mode == IZ_DIAG) {
-            for (int f = 0; f < 6; ++f) a.out[f] = W_.get() + f * M;
-            pt_->inv_z(a, IZ_DIAG, n, H_, s);
-            pt_->lines(W_.get(), 6LL * H_ * n, true, tw_.get(), s);
+        if (mode == PZ_DIAG) {
+            pt_->pinv_z(a, PZ_DIAG, n, H_, s);
+            pt_->plines(W_.get(), 3LL * H_ * n, true, tw_.get(), s);
             for (int c = 0; c < 3; ++c) {
-                x.a[c] = W_.get() + (2 * c) * M;
-                x.b[c] = W_.get() + (2 * c + 1) * M;
+                x.ab[c] = W_.get() + c * M2;
                 x.oa[c] = P_.get() + c * npts_;
                 x.ob[c] = P_.get() + (3 + c) * npts_;
             }
             x.npairs = 3;
         } else {
-            for (int f = 0; f < 3; ++f) a.out[f] = W_.get() + f * M;
-            a.nfields = 3;
-            pt_->inv_z(a, IZ_PLAIN, n, H_, s);
-            pt_->lines(W_.get(), 3LL * H_ * n, true, tw_.get(), s);
-            x.a[0] = W_.get();
-            x.b[0] = W_.get() + M;
+            pt_->pinv_z(a, PZ_VEL, n, H_, s);
+            pt_->plines(W_.get(), 2LL * H_ * n, true, tw_.get(), s);
+            x.ab[0] = W_.get();
             x.oa[0] = P_.get();
             x.ob[0] = P_.get() + npts_;
-            x.a[1] = W_.get() + 2 * M;
-            x.b[1] = nullptr;
+            x.ab[1] = W_.get() + M2;
             x.oa[1] = P_.get() + 2 * npts_;
             x.ob[1] = nullptr;
             x.npairs = 2;
         }
-        pt_->inv_x(x, n, n, s);
+        pt_->pinv_x(x, n, n, s);
         hip_check(hipGetLastError(), "inverse transform launch");
     }
 
@@ -634,10 +628,8 @@ private:
         pt_->fwd_x(fx, FX_PLAIN, n, n, s);
         pt_->lines(W_.get(), 3LL * H_ * n, false, tw_.get(), s);
         FwdZArgs<T> fz{};
-        for (int c = 0; c < 3; ++c) {
-            fz.in[c] = W_.get() + c * M;
-            fz.out[c] = U_.get() + c * M;
-        }
+        for (int c = 0; c < 3; ++c) fz.in[c] = W_.get() + c * M;
+        fz.u3 = U_.get();
         fz.ny = n;
         fz.n = n;
         fz.tw = tw_.get();

@@ -1,0 +1,483 @@
+// Pass kernels of the solver's RK3 stage on pair-interleaved work buffers.
+//
+// Everything the nonlinear term transforms comes in pairs: (u_hat, u_hat*phi)
+// on the inverse side (unshifted / half-cell-shifted velocity,
+// src/ava_taylor_green_impl.hpp:851-879), (u_a u_b, s_a s_b) on the forward
+// side (impl.hpp:903-911), (u_hat, omega_hat) for the diagnostics
+// (impl.hpp:1129-1160). The work buffers store the two members of a pair side
+// by side per mode:
+//   pair field  [kx][kz or z][ky][2]        offset ((kx*n + kz)*ny + ky)*2 + f
+// and the state is component-interleaved:
+//   u_hat, g    [kx][kz][ky][3]             offset ((kx*n + kz)*ny + ky)*3 + c.
+// In the z passes a tile is NL/2 adjacent ky times both pair members, i.e. one
+// 128-byte row segment per element; lane l transforms member f = l & 1 of line
+// ky = l >> 1, and the two lanes of a line split the modes after the
+// transform (a lane swap), so each lane keeps RHS accumulators for half of
+// its modes only.
+#pragma once
+
+#include "passes.hpp"
+
+namespace sph {
+
+enum PairZMode { PZ_RHS = 0, PZ_DIAG = 1, PZ_VEL = 2 };
+
+template <typename T>
+struct PairInvZArgs {
+    const cx<T>* u3;  // state, component-interleaved
+    cx<T>* out[3];    // pair fields
+    int ny;
+    int n;
+    const cx<T>* phase;
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct PairInvXArgs {
+    const cx<T>* ab[3];  // pair fields (A, B) in mixed space [kx][z][y][2]
+    T* oa[3];
+    T* ob[3];  // may be null (B discarded)
+    int npairs;
+    int ny;
+    int nz;
+    T scale;
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct PairFwdXArgs {
+    const T* in[6];  // u0 u1 u2 s0 s1 s2 (physical)
+    cx<T>* out[6];   // pair fields (P_ab, Q_ab), ab in (00,01,02,11,12,22)
+    int ny;
+    int nz;
+    const cx<T>* tw;
+};
+
+template <typename T>
+struct PairFwdZArgs {
+    const cx<T>* in[6];  // pair fields after the x and y transforms
+    cx<T>* u3;
+    cx<T>* g3;
+    int ny;
+    int n;
+    T nu;
+    const T* coef;
+    int stage;
+    const cx<T>* phase;
+    const cx<T>* tw;
+};
+
+// both members of a pair at one mode: one 16-byte (FP32) / 32-byte access
+template <typename T>
+struct alignas(4 * sizeof(T)) cx2 {
+    cx<T> a;
+    cx<T> b;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
+    return {__shfl_xor(v.x, 1), __shfl_xor(v.y, 1)};
+}
+
+// ---------------------------------------------------------------------------
+// inverse z: state -> pair fields
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a) {
+    using Z = ZTile<T, N>;
+    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    static_assert(NL % 2 == 0, "pair tiles need an even line count");
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const int l = tid % NL, f = l & 1;
+    const int ky = blockIdx.x * (NL / 2) + (l >> 1);
+    const int kx = blockIdx.y;
+    const bool ok = ky < a.ny;
+    const uint32_t kyc = ok ? ky : a.ny - 1;
+    constexpr uint32_t CB = sizeof(cx<T>);
+    const uint32_t ny = a.ny;
+    const size_t plane = static_cast<size_t>(kx) * N * ny;
+    const Buf<T> U(a.u3 + plane * 3, N * ny * 3 * CB);
+    const uint32_t u_lane = ((tid / NL) * ny + kyc) * 3 * CB, u_row = ny * 3 * CB;
+    const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
+    const uint32_t p_bytes = N * ny * 2 * CB;
+    load_twiddles<T, N, NT>(twl, a.tw, tid);
+    cx<T> v[1][T0][R0], w[1][TLS][RL];
+    auto uoff = [&](int i, int r, int c) -> uint32_t {
+        return u_lane + (i * Z::STEP + r * (N / R0)) * u_row + c * CB;
+    };
+    auto store = [&](cx<T>* dst) {
+        if (!ok) return;
+        const Buf<T> b(dst + plane * 2, p_bytes);
+#pragma unroll
+        for (int i = 0; i < TLS; ++i)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) b.st(p_lane + (i * Z::STEP + r * (N / RL)) * p_row, w[0][i][r]);
+    };
+    const int my = signed_mode(kyc, a.n);
+    // all three components of the lane's modes, read once (one 24/48-byte row
+    // per mode of the component-interleaved state)
+    cx<T> us[3][T0][R0];
+#pragma unroll
+    for (int i = 0; i < T0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) us[c][i][r] = U.ld(uoff(i, r, c));
+    if constexpr (MODE == PZ_RHS) {
+        // member 0: u_hat_c, member 1: u_hat_c * phi (impl.hpp:866-873)
+        cx<T> ph[T0][R0];
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int mz = signed_mode(Z::e_first(tid, i, r), a.n);
+                ph[i][r] = f ? a.phase[kx + my + mz + a.n] : cx<T>{T(1), T(0)};
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int i = 0; i < T0; ++i)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) v[0][i][r] = f ? cmul(us[c][i][r], ph[i][r]) : us[c][i][r];
+            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            store(a.out[c]);
+        }
+    } else if constexpr (MODE == PZ_DIAG) {
+        // member 0: u_hat_c, member 1: omega_hat_c = (i k x u_hat)_c (impl.hpp:1147-1157)
+        const T ix = T(kx), iy = T(my);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int i = 0; i < T0; ++i)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const cx<T> u0 = us[0][i][r], u1 = us[1][i][r], u2 = us[2][i][r];
+                    const T iz = T(signed_mode(Z::e_first(tid, i, r), a.n));
+                    cx<T> om;
+                    if (c == 0)
+                        om = {-(iy * u2.y - iz * u1.y), iy * u2.x - iz * u1.x};
+                    else if (c == 1)
+                        om = {-(iz * u0.y - ix * u2.y), iz * u0.x - ix * u2.x};
+                    else
+                        om = {-(ix * u1.y - iy * u0.y), ix * u1.x - iy * u0.x};
+                    const cx<T> uc = c == 0 ? u0 : (c == 1 ? u1 : u2);
+                    v[0][i][r] = f ? om : uc;
+                }
+            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            store(a.out[c]);
+        }
+    } else {  // PZ_VEL: pairs (u0, u1) and (u2, 0)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+#pragma unroll
+            for (int i = 0; i < T0; ++i)
+#pragma unroll
+                for (int r = 0; r < R0; ++r)
+                    v[0][i][r] = q == 0 ? (f ? us[1][i][r] : us[0][i][r]) : (f ? czero<T>() : us[2][i][r]);
+            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            store(a.out[q]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// contiguous y lines of pair fields, in place: a tile is NL/2 pair lines,
+// i.e. NL*N consecutive complex values, member f of pair line g is line 2g+f
+// ---------------------------------------------------------------------------
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__((Tile<T, N>::NT)) k_plines(cx<T>* data, long long npairlines, const cx<T>* tw) {
+    using TL = Tile<T, N>;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
+    static_assert(NL % 2 == 0, "pair tiles need an even line count");
+    __shared__ cx<T> buf[TL::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const long long g0 = static_cast<long long>(blockIdx.x) * (NL / 2);
+    cx<T>* base = data + g0 * 2 * N;
+    const int valid = static_cast<int>(npairlines - g0 < NL / 2 ? npairlines - g0 : NL / 2) * 2 * N;
+    load_twiddles<T, N, NT>(twl, tw, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int m = tid + i * NT;
+        const int g = m / (2 * N), rem = m % (2 * N);
+        if (m < TL::ELEMS) buf[(rem >> 1) * S + 2 * g + (rem & 1)] = m < valid ? base[m] : czero<T>();
+    }
+    __syncthreads();
+    fft_tile<T, N, NL, NT, INV>(buf, twl, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int m = tid + i * NT;
+        const int g = m / (2 * N), rem = m % (2 * N);
+        if (m < valid) base[m] = buf[(rem >> 1) * S + 2 * g + (rem & 1)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse x (complex-to-real) of pair fields: Z = A + iB per row
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) {
+    using TL = Tile<T, N>;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
+    constexpr int H = N / 2 + 1;
+    constexpr int LOADS = NL * H;
+    constexpr int LPT = (LOADS + NT - 1) / NT;
+    __shared__ cx<T> buf[TL::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * NL;
+    const int z = blockIdx.y;
+    load_twiddles<T, N, NT>(twl, a.tw, tid);
+#pragma unroll 1
+    for (int p = 0; p < a.npairs; ++p) {
+        const cx<T>* AB = pick(a.ab, p);
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            const int idx = tid + i * NT;
+            if (LOADS % NT == 0 || idx < LOADS) {
+                const int l = idx % NL, k = idx / NL;
+                cx<T> va = czero<T>(), vb = czero<T>();
+                if (y0 + l < a.ny) {
+                    const size_t off = (static_cast<size_t>(k) * a.nz + z) * a.ny + y0 + l;
+                    const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
+                    va = q.a;
+                    vb = q.b;
+                }
+                if (k == 0 || k == N / 2) {
+                    va.y = T(0);
+                    vb.y = T(0);
+                }
+                buf[k * S + l] = {va.x - vb.y, va.y + vb.x};
+                if (k > 0 && k < N / 2) buf[(N - k) * S + l] = {va.x + vb.y, vb.x - va.y};
+            }
+        }
+        __syncthreads();
+        fft_tile<T, N, NL, NT, true>(buf, twl, tid);
+        T* OA = pick(a.oa, p);
+        T* OB = pick(a.ob, p);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int idx = tid + i * NT;
+            const int x = idx % N, l = idx / N;
+            if (idx < TL::ELEMS && y0 + l < a.ny) {
+                const cx<T> v = buf[x * S + l];
+                const int o = (z * a.ny + y0 + l) * N + x;
+                OA[o] = v.x * a.scale;
+                if (OB) OB[o] = v.y * a.scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward x (real-to-complex) of the six product pairs
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__((Tile<T, N>::NT)) k_pfwd_x(PairFwdXArgs<T> a) {
+    using TL = Tile<T, N>;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
+    constexpr int H = N / 2 + 1;
+    constexpr int ITEMS = NL * H;
+    constexpr int IPT = (ITEMS + NT - 1) / NT;
+    __shared__ cx<T> buf[TL::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * NL;
+    const int z = blockIdx.y;
+    load_twiddles<T, N, NT>(twl, a.tw, tid);
+    T u[3][E], s[3][E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int idx = tid + i * NT;
+        const int x = idx % N, l = idx / N;
+        const bool ok = idx < TL::ELEMS && y0 + l < a.ny;
+        const int o = (z * a.ny + (ok ? y0 + l : 0)) * N + x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            u[c][i] = ok ? a.in[c][o] : T(0);
+            s[c][i] = ok ? a.in[3 + c][o] : T(0);
+        }
+    }
+    constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int idx = tid + i * NT;
+            if (TL::ELEMS % NT == 0 || idx < TL::ELEMS)
+                buf[(idx % N) * S + idx / N] = {u[PA[p]][i] * u[PB[p]][i], s[PA[p]][i] * s[PB[p]][i]};
+        }
+        __syncthreads();
+        fft_tile<T, N, NL, NT, false>(buf, twl, tid);
+        cx<T>* out = a.out[p];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const int idx = tid + i * NT;
+            if (ITEMS % NT == 0 || idx < ITEMS) {
+                const int l = idx % NL, k = idx / NL;
+                if (y0 + l < a.ny) {
+                    const cx<T> zk = buf[k * S + l];
+                    const cx<T> zm = buf[((N - k) % N) * S + l];
+                    const size_t off = (static_cast<size_t>(k) * a.nz + z) * a.ny + y0 + l;
+                    cx2<T> q;
+                    q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+                    q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+                    reinterpret_cast<cx2<T>*>(out)[off] = q;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward z of the six product pairs + RHS completion + RK3 update
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T> a) {
+    using Z = ZTile<T, N>;
+    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    constexpr int SL = TLS * RL;  // last-stage slots per lane
+    constexpr int HS = SL / 2;    // modes owned by each lane of a line
+    static_assert(SL % 2 == 0, "the two lanes of a line split the slots");
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const int l = tid % NL, f = l & 1;
+    const int ky = blockIdx.x * (NL / 2) + (l >> 1);
+    const int kx = blockIdx.y;
+    const bool ok = ky < a.ny;
+    const uint32_t kyc = ok ? ky : a.ny - 1;
+    constexpr uint32_t CB = sizeof(cx<T>);
+    const uint32_t ny = a.ny;
+    const size_t plane = static_cast<size_t>(kx) * N * ny;
+    const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
+    const uint32_t p_bytes = N * ny * 2 * CB;
+    load_twiddles<T, N, NT>(twl, a.tw, tid);
+    const int my = signed_mode(kyc, a.n);
+    const T fx = T(kx), fy = T(my);
+    // element (kz) of owned mode h of this lane: slot h (f = 0) or h + HS (f = 1)
+    auto owned_e = [&](int h) {
+        const int lo = Z::e_last(tid, h / RL, h % RL), hi = Z::e_last(tid, (h + HS) / RL, (h + HS) % RL);
+        return f ? hi : lo;
+    };
+    cx<T> r0[HS], r1[HS], r2[HS];
+#pragma unroll
+    for (int h = 0; h < HS; ++h) r0[h] = r1[h] = r2[h] = czero<T>();
+    cx<T> v[1][T0][R0], w[1][TLS][RL], vn[T0][R0];
+    auto load_pair = [&](int p, cx<T> (&dst)[T0][R0]) {
+        const Buf<T> b(pick(a.in, p) + plane * 2, p_bytes);
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) dst[i][r] = b.ld(p_lane + (i * Z::STEP + r * (N / R0)) * p_row);
+    };
+    load_pair(0, vn);
+    // products in the reference's accumulation order (00,01,02,11,12,22);
+    // the next pair's loads are issued before the current pair's transform
+#pragma unroll 1
+    for (int p = 0; p < 6; ++p) {
+        const int pa = p < 3 ? 0 : (p < 5 ? 1 : 2);
+        const int pb = p < 3 ? p : (p < 5 ? p - 2 : 2);
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[0][i][r] = vn[i][r];
+        if (p < 5) load_pair(p + 1, vn);
+        zfft<T, N, 1, false>(v, w, lds, twl, tid);
+        // lane f = 0 holds P_hat, f = 1 holds Q_hat of every slot; swap halves
+#pragma unroll
+        for (int h = 0; h < HS; ++h) {
+            const cx<T> lo = w[0][h / RL][h % RL], hi = w[0][(h + HS) / RL][(h + HS) % RL];
+            const cx<T> mine = f ? hi : lo;
+            const cx<T> other = shfl_xor1(f ? lo : hi);
+            const cx<T> P = f ? other : mine;
+            const cx<T> Q = f ? mine : other;
+            const int mz = signed_mode(owned_e(h), a.n);
+            const int k2 = kx * kx + my * my + mz * mz;
+            const cx<T> ph = a.phase[kx + my + mz + a.n];
+            cx<T> val = (P + cmul(Q, cx<T>{ph.x, -ph.y})) * T(0.5);
+            if (!keep_mode(k2, a.n)) val = czero<T>();
+            const T fz = T(mz);
+            const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
+            const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
+            const cx<T> da = {kb * val.y, -kb * val.x};  // -i k_b v into r_a
+            const cx<T> db = {ka * val.y, -ka * val.x};  // -i k_a v into r_b
+            if (pa == 0)
+                r0[h] = r0[h] + da;
+            else if (pa == 1)
+                r1[h] = r1[h] + da;
+            else
+                r2[h] = r2[h] + da;
+            if (pa != pb) {
+                if (pb == 1)
+                    r1[h] = r1[h] + db;
+                else
+                    r2[h] = r2[h] + db;
+            }
+        }
+    }
+    if (!ok) return;
+    const T alpha = a.coef[2 * a.stage];
+    const T beta = a.coef[2 * a.stage + 1];
+    const bool first = a.stage == 0;
+    const Buf<T> U(a.u3 + plane * 3, N * ny * 3 * CB);
+    const Buf<T> G(a.g3 + plane * 3, N * ny * 3 * CB);
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+        const int e = owned_e(h);
+        const uint32_t off = (e * ny + kyc) * 3 * CB;
+        const int mz = signed_mode(e, a.n);
+        const int k2 = kx * kx + my * my + mz * mz;
+        const bool keep = keep_mode(k2, a.n);
+        cx<T> u[3] = {U.ld(off), U.ld(off + CB), U.ld(off + 2 * CB)};
+        cx<T> rr[3] = {r0[h], r1[h], r2[h]};
+        const T vk2 = a.nu * T(k2);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[c] * vk2;
+        project_or_zero(rr[0], rr[1], rr[2], kx, my, mz, k2, keep);
+        cx<T> g[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            g[c] = first ? rr[c] : G.ld(off + c * CB) * alpha + rr[c];
+            u[c] = u[c] + g[c] * beta;
+        }
+        project_or_zero(u[0], u[1], u[2], kx, my, mz, k2, keep);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G.st(off + c * CB, g[c]);
+            U.st(off + c * CB, u[c]);
+        }
+    }
+}
+
+// max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)
+template <typename T>
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_div3(const cx<T>* u3, int n, double* partial) {
+    const long long count = static_cast<long long>(n / 2 + 1) * n * n;
+    double m = 0.0;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int ky = static_cast<int>(i % n);
+        const int kz = static_cast<int>((i / n) % n);
+        const int kx = static_cast<int>(i / (static_cast<long long>(n) * n));
+        const T fx = T(kx), fy = T(signed_mode(ky, n)), fz = T(signed_mode(kz, n));
+        const cx<T> d = (u3[3 * i] * fx + u3[3 * i + 1] * fy) + u3[3 * i + 2] * fz;
+        m = fmax(m, static_cast<double>(sqrt(d.x * d.x + d.y * d.y)));
+    }
+    __shared__ double red[kReduceThreads / 64];
+    m = wave_max(m);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < kReduceThreads / 64; ++w) b = fmax(b, red[w]);
+        partial[blockIdx.x] = b;
+    }
+}
+
+}  // namespace sph

@@ -3,7 +3,7 @@
 // passes_f32.hip / passes_f64.hip.
 #pragma once
 
-#include "passes.hpp"
+#include "pair_passes.hpp"
 
 namespace sph {
 
@@ -17,6 +17,12 @@ struct PassTable {
     void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
     void (*fwd_x)(const FwdXArgs<T>&, int mode, int ny, int nz, hipStream_t);
     void (*fwd_z)(const FwdZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
+    // solver stage on pair-interleaved buffers
+    void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
+    void (*plines)(cx<T>* data, long long npairlines, bool inverse, const cx<T>* tw, hipStream_t);
+    void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, hipStream_t);
+    void (*pfwd_x)(const PairFwdXArgs<T>&, int ny, int nz, hipStream_t);
+    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -66,7 +72,36 @@ struct Launch {
         else
             k_fwd_z<T, N, FZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static constexpr PassTable<T> table{N, TL::NL, TL::NT, &inv_z, &lines, &inv_x, &fwd_x, &fwd_z};
+    static void pinv_z(const PairInvZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
+        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx);
+        if (mode == PZ_RHS)
+            k_pinv_z<T, N, PZ_RHS><<<grid, ZT::NT, 0, s>>>(a);
+        else if (mode == PZ_DIAG)
+            k_pinv_z<T, N, PZ_DIAG><<<grid, ZT::NT, 0, s>>>(a);
+        else
+            k_pinv_z<T, N, PZ_VEL><<<grid, ZT::NT, 0, s>>>(a);
+    }
+    static void plines(cx<T>* data, long long npairlines, bool inverse, const cx<T>* tw, hipStream_t s) {
+        const dim3 grid(static_cast<unsigned>((npairlines + TL::NL / 2 - 1) / (TL::NL / 2)));
+        if (inverse)
+            k_plines<T, N, true><<<grid, TL::NT, 0, s>>>(data, npairlines, tw);
+        else
+            k_plines<T, N, false><<<grid, TL::NT, 0, s>>>(data, npairlines, tw);
+    }
+    static void pinv_x(const PairInvXArgs<T>& a, int ny, int nz, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+        k_pinv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static void pfwd_x(const PairFwdXArgs<T>& a, int ny, int nz, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+        k_pfwd_x<T, N><<<grid, TL::NT, 0, s>>>(a);
+    }
+    static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, hipStream_t s) {
+        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx);
+        k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
+    }
+    static constexpr PassTable<T> table{N,     TL::NL,  TL::NT,  &inv_z,  &lines,  &inv_x,    &fwd_x,
+                                        &fwd_z, &pinv_z, &plines, &pinv_x, &pfwd_x, &pfwd_z_rk};
 };
 
 }  // namespace sph

@@ -124,6 +124,7 @@ template <typename T>
 struct FwdZArgs {
     const cx<T>* in[12];
     cx<T>* out[3];  // PLAIN / PROJECT outputs
+    cx<T>* u3;      // PROJECT: if set, the output is this component-interleaved state
     cx<T>* u[3];    // RK: state (read-write)
     cx<T>* g[3];    // RK: low-storage register (read-write)
     int nfields;    // PLAIN
@@ -481,8 +482,15 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
                     const int mz = signed_mode(e, a.n);
                     const int k2 = kx * kx + my * my + mz * mz;
                     project_or_zero(u[0][i][r], u[1][i][r], u[2][i][r], kx, my, mz, k2, keep_mode(k2, a.n));
+                    if (a.u3) {
+                        const Buf<T> o(a.u3 + plane * 3, plane_bytes * 3);
+                        const uint32_t off3 = ((Z::e_last(tid, i, r)) * a.ny + ky) * 3 * sizeof(cx<T>);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) Buf<T>(a.out[c] + plane, plane_bytes).st(last_off(i, r), u[c][i][r]);
+                        for (int c = 0; c < 3; ++c) o.st(off3 + c * sizeof(cx<T>), u[c][i][r]);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) Buf<T>(a.out[c] + plane, plane_bytes).st(last_off(i, r), u[c][i][r]);
+                    }
                 }
         }
     } else {  // FZ_RK
