@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PASS_NAMES = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
-KERNEL_NAMES = ("k_inv_z", "k_lines", "k_inv_x", "k_fwd_x", "k_lines", "k_fwd_z")
+KERNEL_NAMES = ("k_pinv_z", "k_plines", "k_pinv_x", "k_pfwd_x", "k_plines", "k_pfwd_z_rk")
+ENSEMBLE_SIZES = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
 
 
 def log(*a):
@@ -52,6 +53,34 @@ def time_steps(sph, n, precision, steps, warmup, dt):
     s.synchronize()
     sec = (time.perf_counter() - t0) / steps
     return s, sec
+
+
+def ensemble_leg(sph, steps: int, warmup: int):
+    """64 independent solves (16 each of 32^3, 48^3, 64^3, 96^3), FP32,
+    nu = 1e-2, seeded random-phase solenoidal initial fields (seed 1000 + j,
+    E0 = 0.5), dt fixed per member from CFL 0.5 at t = 0."""
+    import math
+    sizes = ENSEMBLE_SIZES
+    ens = sph.Ensemble(sizes, 1e-2, [1000 + j for j in range(len(sizes))], precision="fp32")
+    dt = [0.5 * (2 * math.pi / n) / ens.max_velocity(j) for j, n in enumerate(sizes)]
+    for _ in range(warmup):
+        ens.step_rk3(dt)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step_rk3(dt)
+    ens.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    pts = sum(n ** 3 for n in sizes)
+    alg = sum(step_bytes(n, 4) for n in sizes)
+    passes = ens.time_passes(3)
+    ens.close()
+    return {"workload": "64 solves: 16 each of 32^3, 48^3, 64^3, 96^3, FP32, nu=1e-2, CFL 0.5 dt, "
+                        f"{steps} RK3 steps, one hipGraph per step",
+            "mpts_per_s": round(pts / sec / 1e6, 3), "ms_per_step": round(sec * 1e3, 4),
+            "points": pts, "algorithmic_bytes_per_step": alg,
+            "roofline_frac": round(alg / sec / 1e9 / HBM_PEAK_GBS, 4),
+            "passes_ms": {k: round(v[0], 4) for k, v in passes.items()}}
 
 
 def traffic_from_profiles(kernel: str, n: int):
@@ -86,6 +115,8 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="launches per pass for the kernel timing")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the workload is one {args.n}^3 grid per GPU and this "
@@ -126,6 +157,17 @@ def main():
                 "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()}}
     s32.close()
 
+    copy_gbs = sph.copy_bandwidth(2 << 30, 10)
+    log(f"copy kernel: {copy_gbs:.1f} GB/s")
+    roofline["copy_ceiling_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+
+    ens = None
+    if not args.no_ensemble:
+        log(f"ensemble: 64 members, {args.ensemble_steps} steps")
+        ens = ensemble_leg(sph, args.ensemble_steps, W)
+        log("ensemble:", ens)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -152,6 +194,7 @@ def main():
         "dissipation_rel_err_fp32_vs_fp64": eps_err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     if fp64:
         out["energy_rel_err_fp32_vs_fp64"] = e_err

@@ -159,6 +159,12 @@ int sph_solver_synchronize(sph_solver_t solver);
  * each pass in bytes[6]. The state is unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
+/* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,
+ * timed with HIP events over `reps` launches; *gbs = 2*bytes/time (read +
+ * write) in GB/s. The measured ceiling against which the pass kernels are
+ * compared, and the calibration kernel for hardware byte counters. */
+int sph_measure_copy_bandwidth(int64_t bytes, int reps, double* gbs);
+
 /* run_taylor_green(config) (impl.hpp:1310-1410): fixed-dt or adaptive-CFL
  * driver writing <output_dir>/taylor_green_diagnostics.csv. The fields mirror
  * RunConfig (src/ava_taylor_green.hpp:21-34); precision selects FP32/FP64.
@@ -180,6 +186,45 @@ typedef struct sph_run_config {
 } sph_run_config;
 
 int sph_run_taylor_green(const sph_run_config* config, sph_diagnostics* final_row);
+
+/* ------------------------------------------------------------------------
+ * Batched ensemble of independent periodic solves.
+ *
+ * The reference runs one TaylorGreenSolver per process (impl.hpp:658-1308);
+ * this is that solver batched: `count` members, each with its own grid size
+ * n[j] (any compiled length), viscosity nu[j] and seeded random-phase
+ * solenoidal initial field with E(k) ~ k^4 exp(-2 (k/4)^2), scaled to kinetic
+ * energy `energy`. Members of equal size share every pass launch (one launch
+ * per pass per RK stage per distinct size), and a step is one hipGraph.
+ * Each member's state follows the same RK3 stage as TaylorGreenSolver::step.
+ * ------------------------------------------------------------------------ */
+typedef struct sph_ensemble_s* sph_ensemble_t;
+
+int sph_ensemble_create(int count, const int* n, const double* nu, const uint64_t* seeds, double energy,
+                        int precision, sph_ensemble_t* out);
+int sph_ensemble_destroy(sph_ensemble_t ens);
+
+/* One RK3 step of every member; dt[j] is member j's time step. */
+int sph_ensemble_step_rk3(sph_ensemble_t ens, const double* dt);
+
+/* Same meaning as the solver calls, for member `member`. */
+int sph_ensemble_diagnostics(sph_ensemble_t ens, int member, double t, sph_diagnostics* out);
+int sph_ensemble_max_velocity(sph_ensemble_t ens, int member, double* out);
+int sph_ensemble_get_uhat(sph_ensemble_t ens, int member, int component, double* host);
+int sph_ensemble_set_uhat(sph_ensemble_t ens, int member, int component, const double* host);
+int sph_ensemble_set_graph(sph_ensemble_t ens, int enabled);
+int sph_ensemble_synchronize(sph_ensemble_t ens);
+
+/* GPU seconds spent in sph_ensemble_step_rk3 and the number of steps. */
+int sph_ensemble_step_seconds(sph_ensemble_t ens, double* seconds, int64_t* steps);
+
+/* Pass timings as sph_solver_time_passes, summed over the distinct sizes. */
+int sph_ensemble_time_passes(sph_ensemble_t ens, int reps, double* ms, double* bytes);
+
+/* The ensemble's initial field for one member, on the host: three
+ * components of n*n*(n/2+1) interleaved (re, im) doubles, layout as
+ * sph_solver_get_uhat. */
+int sph_random_solenoidal_field(int n, uint64_t seed, double energy, double* u0, double* u1, double* u2);
 
 #ifdef __cplusplus
 }

@@ -185,3 +185,78 @@ def c2c_to_half(a: np.ndarray) -> np.ndarray:
     [kx in 0..n/2][kz][ky] used by the GPU library."""
     nx = a.shape[2]
     return np.ascontiguousarray(np.transpose(a[:, :, : nx // 2 + 1], (2, 0, 1)))
+
+
+def half_to_c2c(h: np.ndarray) -> np.ndarray:
+    """Inverse of c2c_to_half for a Hermitian spectrum: the full C2C
+    [kz, ky, kx] array whose kx > n/2 half is conj(u_hat(n - kx, -ky, -kz))."""
+    nh, nz, ny = h.shape
+    nx = 2 * (nh - 1)
+    full = np.zeros((nz, ny, nx), np.complex128)
+    full[:, :, :nh] = np.transpose(h, (1, 2, 0))
+    kx = np.arange(nh, nx)
+    src = full[:, :, nx - kx]  # [kz, ky, kx']
+    src = np.roll(np.flip(src, axis=(0, 1)), shift=(1, 1), axis=(0, 1))  # (-kz, -ky)
+    full[:, :, nh:] = np.conj(src)
+    return full
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser (Steele, Lea & Flood 2014) on uint64 arrays."""
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def random_solenoidal_field(n: int, seed: int, energy: float = 0.5):
+    """numpy restatement of the ensemble's initial field (three complex
+    [kx][kz][ky] half-spectrum arrays).
+
+    Every kept mode (4|m|^2 <= n^2, no Nyquist index, m != 0; on the kx = 0
+    plane only the half my > 0 or (my = 0, mz > 0), the other half being its
+    conjugate) gets three unit phasors e^{i theta_c}, theta_c = 2 pi r / 2^53
+    with r = SplitMix64(seed * 0x2545F4914F6CDD1D + 3 key + c) >> 11 and
+    key = ((mx + n)(2n + 1) + my + n)(2n + 1) + mz + n; the vector is projected
+    perpendicular to k and scaled to |u_hat|^2 = E(k) / (4 pi k^2) with
+    E(k) = k^4 exp(-2 (k/4)^2); the whole field is then scaled to
+    (1/2) sum_full |u_hat|^2 / n^6 = energy.
+    """
+    H = n // 2 + 1
+    kx = np.arange(H)[:, None, None]
+    sg = np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+    mz = sg[None, :, None]
+    my = sg[None, None, :]
+    mx, my, mz = np.broadcast_arrays(kx, my, mz)
+    k2 = mx * mx + my * my + mz * mz
+    gen = (k2 > 0) & (4 * k2 <= n * n) & (mx != n // 2) & (my != n // 2) & (mz != n // 2)
+    gen &= (mx > 0) | (my > 0) | ((my == 0) & (mz > 0))
+    key = ((mx + n).astype(np.uint64) * np.uint64(2 * n + 1) + (my + n).astype(np.uint64)) * np.uint64(2 * n + 1) \
+        + (mz + n).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) * np.uint64(0x2545F4914F6CDD1D) + key * np.uint64(3)
+    v = []
+    for c in range(3):
+        with np.errstate(over="ignore"):
+            r = _splitmix64(base + np.uint64(c))
+        th = 2.0 * np.pi * (r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        v.append(np.cos(th) + 1j * np.sin(th))
+    kk = [mx.astype(np.float64), my.astype(np.float64), mz.astype(np.float64)]
+    k2f = np.where(gen, k2, 1).astype(np.float64)
+    dot = kk[0] * v[0] + kk[1] * v[1] + kk[2] * v[2]
+    v = [v[c] - kk[c] * dot / k2f for c in range(3)]
+    norm = sum(np.abs(x) ** 2 for x in v)
+    gen &= norm >= 1e-20
+    k = np.sqrt(k2f)
+    ek = k ** 4 * np.exp(-2.0 * (k / 4.0) ** 2)
+    amp = np.where(gen, np.sqrt(ek / (4.0 * np.pi * k * k) / np.where(gen, norm, 1.0)), 0.0)
+    u = [np.where(gen, amp * v[c], 0.0) for c in range(3)]
+    # kx = 0 plane: partner (-my, -mz) is the conjugate
+    for c in range(3):
+        p = u[c][0]
+        part = np.conj(np.roll(np.flip(p, axis=(0, 1)), shift=(1, 1), axis=(0, 1)))
+        u[c][0] = np.where(gen[0], p, part)
+    total = 2.0 * sum(np.sum(np.where(gen, np.abs(u[c]) ** 2, 0.0)) for c in range(3))
+    scale = np.sqrt(energy * float(n) ** 6 / (0.5 * total))
+    return [np.ascontiguousarray(x * scale) for x in u]

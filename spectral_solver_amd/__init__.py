@@ -8,6 +8,8 @@ include/spectral_hip.h) with the reference's class shapes:
                            (src/ava_taylor_green.hpp:36-53)
     Diagnostics, RunConfig, run_taylor_green
                         <- src/ava_taylor_green.hpp:8-34, 55
+    Ensemble            a batch of independent TaylorGreenSolver-style solves
+                        (uneven sizes) stepped in shared launches
 
 Every compute call goes to the HIP library; if the library has not been built
 (``python -m spectral_solver_amd.build``) the first use raises.
@@ -21,8 +23,9 @@ from typing import Callable, Optional
 
 import numpy as np
 
-__all__ = ["HipFft3D", "TaylorGreenSolver", "Diagnostics", "RunConfig", "run_taylor_green",
-           "SpectralMode", "library_path", "load_library", "device_count", "FP32", "FP64"]
+__all__ = ["HipFft3D", "TaylorGreenSolver", "Diagnostics", "RunConfig", "run_taylor_green", "Ensemble",
+           "random_solenoidal_field", "SpectralMode", "library_path", "load_library", "device_count",
+           "copy_bandwidth", "FP32", "FP64"]
 
 FP32 = 0
 FP64 = 1
@@ -83,7 +86,21 @@ SYMBOLS = [
     ("sph_solver_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sph_solver_synchronize", ctypes.c_int, [_VP]),
     ("sph_solver_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
+    ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
+    ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.c_double, ctypes.c_int, ctypes.POINTER(_VP)]),
+    ("sph_ensemble_destroy", ctypes.c_int, [_VP]),
+    ("sph_ensemble_step_rk3", ctypes.c_int, [_VP, _DP]),
+    ("sph_ensemble_diagnostics", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_Diag)]),
+    ("sph_ensemble_max_velocity", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
+    ("sph_ensemble_get_uhat", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
+    ("sph_ensemble_set_uhat", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
+    ("sph_ensemble_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sph_ensemble_synchronize", ctypes.c_int, [_VP]),
+    ("sph_ensemble_step_seconds", ctypes.c_int, [_VP, _DP, _I64P]),
+    ("sph_ensemble_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
+    ("sph_random_solenoidal_field", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _DP, _DP, _DP]),
 ]
 
 
@@ -115,6 +132,13 @@ def device_count() -> int:
     c = ctypes.c_int()
     _check(load_library().sph_device_count(ctypes.byref(c)))
     return c.value
+
+
+def copy_bandwidth(nbytes: int = 2 << 30, reps: int = 10) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes)."""
+    g = ctypes.c_double()
+    _check(load_library().sph_measure_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 def _precision(p) -> int:
@@ -332,8 +356,7 @@ class TaylorGreenSolver:
         ms = np.zeros(6)
         by = np.zeros(6)
         _check(load_library().sph_solver_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
-        names = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
-        return {k: (float(m), float(b)) for k, m, b in zip(names, ms, by)}
+        return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 
     def u_hat(self, c: int) -> np.ndarray:
         """Component c as complex [kx in 0..n/2][kz][ky]."""
@@ -348,6 +371,113 @@ class TaylorGreenSolver:
         if a.shape != (n // 2 + 1, n, n):
             raise ValueError("u_hat component must have shape (n/2+1, n, n)")
         _check(load_library().sph_solver_set_uhat(self._h, int(c), _dptr(a.view(np.float64))))
+
+
+_PASS_NAMES = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
+
+
+def random_solenoidal_field(n: int, seed: int, energy: float = 0.5):
+    """The ensemble's seeded initial field: three complex [kx][kz][ky] arrays.
+
+    Random phases, |u_hat|^2 shaped to E(k) ~ k^4 exp(-2 (k/4)^2), divergence
+    free, kept modes only, kinetic energy `energy`. Host-side (no GPU).
+    """
+    n = int(n)
+    u = [np.empty((n // 2 + 1, n, n), np.complex128) for _ in range(3)]
+    _check(load_library().sph_random_solenoidal_field(n, ctypes.c_uint64(int(seed)), float(energy),
+                                                      *[_dptr(a.view(np.float64)) for a in u]))
+    return u
+
+
+class Ensemble:
+    """A batch of independent periodic solves stepped together.
+
+    Members may differ in grid size; members of one size share every pass
+    launch and the whole step is one hipGraph. Each member takes the same RK3
+    step as TaylorGreenSolver (impl.hpp:976-1039) from a seeded random-phase
+    solenoidal initial field.
+    """
+
+    def __init__(self, sizes, nu, seeds=None, energy: float = 0.5, precision="fp32"):
+        sizes = [int(s) for s in sizes]
+        count = len(sizes)
+        nu = np.broadcast_to(np.asarray(nu, np.float64), (count,)).copy()
+        seeds = list(range(count)) if seeds is None else [int(s) for s in seeds]
+        if len(seeds) != count:
+            raise ValueError("one seed per member")
+        self.sizes = sizes
+        self.nu = nu
+        self.precision = _precision(precision)
+        n_arr = (ctypes.c_int * count)(*sizes)
+        s_arr = (ctypes.c_uint64 * count)(*seeds)
+        h = _VP()
+        _check(load_library().sph_ensemble_create(count, n_arr, _dptr(nu), s_arr, float(energy), self.precision,
+                                                  ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().sph_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def step_rk3(self, dt) -> None:
+        d = np.broadcast_to(np.asarray(dt, np.float64), (len(self.sizes),)).copy()
+        _check(load_library().sph_ensemble_step_rk3(self._h, _dptr(d)))
+
+    def diagnostics(self, member: int, t: float = 0.0) -> Diagnostics:
+        d = _Diag()
+        _check(load_library().sph_ensemble_diagnostics(self._h, int(member), float(t), ctypes.byref(d)))
+        return Diagnostics._from(d)
+
+    def max_velocity(self, member: int) -> float:
+        v = ctypes.c_double()
+        _check(load_library().sph_ensemble_max_velocity(self._h, int(member), ctypes.byref(v)))
+        return v.value
+
+    def u_hat(self, member: int, c: int) -> np.ndarray:
+        n = self.sizes[member]
+        out = np.empty((n // 2 + 1, n, n), np.complex128)
+        _check(load_library().sph_ensemble_get_uhat(self._h, int(member), int(c), _dptr(out.view(np.float64))))
+        return out
+
+    def set_u_hat(self, member: int, c: int, a: np.ndarray) -> None:
+        n = self.sizes[member]
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != (n // 2 + 1, n, n):
+            raise ValueError("u_hat component must have shape (n/2+1, n, n)")
+        _check(load_library().sph_ensemble_set_uhat(self._h, int(member), int(c), _dptr(a.view(np.float64))))
+
+    def set_graph(self, enabled: bool) -> None:
+        _check(load_library().sph_ensemble_set_graph(self._h, 1 if enabled else 0))
+
+    def synchronize(self) -> None:
+        _check(load_library().sph_ensemble_synchronize(self._h))
+
+    def step_seconds(self):
+        s, k = ctypes.c_double(), ctypes.c_int64()
+        _check(load_library().sph_ensemble_step_seconds(self._h, ctypes.byref(s), ctypes.byref(k)))
+        return s.value, k.value
+
+    def time_passes(self, reps: int = 5):
+        ms = np.zeros(6)
+        by = np.zeros(6)
+        _check(load_library().sph_ensemble_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
+        return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 
 
 def run_taylor_green(config: RunConfig) -> Diagnostics:

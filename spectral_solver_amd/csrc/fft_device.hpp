@@ -81,6 +81,29 @@ constexpr RadixPlan radix_plan(int n) {
 
 constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 0; }
 
+// Plan of the register-direct z passes, where every thread holds lcm(radices)
+// elements per field: a radix-8 stage next to a radix-3 one would make that 24,
+// so there each 8 is split into 4*2 (12 elements per thread).
+constexpr RadixPlan z_radix_plan(int n) {
+    const RadixPlan p = radix_plan(n);
+    bool has8 = false, has3 = false;
+    for (int s = 0; s < p.count; ++s) {
+        has8 |= p.r[s] == 8;
+        has3 |= p.r[s] == 3;
+    }
+    if (!(has8 && has3)) return p;
+    RadixPlan q{};
+    for (int s = 0; s < p.count; ++s) {
+        if (p.r[s] == 8) {
+            q.r[q.count++] = 4;
+            q.r[q.count++] = 2;
+        } else {
+            q.r[q.count++] = p.r[s];
+        }
+    }
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // Tile geometry per (storage type, length).
 // ---------------------------------------------------------------------------
@@ -239,7 +262,7 @@ __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) 
 // ---------------------------------------------------------------------------
 template <typename T, int N>
 struct ZTile {
-    static constexpr RadixPlan P = radix_plan(N);
+    static constexpr RadixPlan P = z_radix_plan(N);
     static constexpr int lcm_radix() {
         int l = 1;
         for (int s = 0; s < P.count; ++s) {
@@ -320,7 +343,7 @@ __device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid
 // stages S .. count-2; NS is the product of the radices before stage S
 template <typename T, int N, int NL, int NT, int NF, bool INV, int S, int NS>
 __device__ __forceinline__ void zstages_mid(cx<T>* lds, const cx<T>* twl, int tid) {
-    constexpr RadixPlan P = radix_plan(N);
+    constexpr RadixPlan P = z_radix_plan(N);
     if constexpr (S <= P.count - 2) {
         zstage_mid<T, N, NL, NT, NF, P.r[S], NS, INV>(lds, twl, tid);
         zstages_mid<T, N, NL, NT, NF, INV, S + 1, NS * P.r[S]>(lds, twl, tid);

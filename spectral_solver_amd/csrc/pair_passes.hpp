@@ -22,6 +22,11 @@ namespace sph {
 
 enum PairZMode { PZ_RHS = 0, PZ_DIAG = 1, PZ_VEL = 2 };
 
+// Batched launches: blockIdx.z (blockIdx.y for the y lines) is a member of an
+// ensemble of equal-size solves; every pointer below addresses member 0 and
+// member m adds m times the stride of its family (ms_u: state, ms_w: work
+// pairs, ms_p: physical fields), all in complex (state, work) or real
+// (physical) elements. A single solve uses zero strides.
 template <typename T>
 struct PairInvZArgs {
     const cx<T>* u3;  // state, component-interleaved
@@ -30,6 +35,8 @@ struct PairInvZArgs {
     int n;
     const cx<T>* phase;
     const cx<T>* tw;
+    long long ms_u;
+    long long ms_w;
 };
 
 template <typename T>
@@ -42,6 +49,8 @@ struct PairInvXArgs {
     int nz;
     T scale;
     const cx<T>* tw;
+    long long ms_w;
+    long long ms_p;
 };
 
 template <typename T>
@@ -51,7 +60,13 @@ struct PairFwdXArgs {
     int ny;
     int nz;
     const cx<T>* tw;
+    long long ms_p;
+    long long ms_w;
 };
+
+// coef: per member 8 values in device memory: alpha_s, beta_s*dt for the
+// three stages, then nu.
+constexpr int kCoefStride = 8;
 
 template <typename T>
 struct PairFwdZArgs {
@@ -60,11 +75,12 @@ struct PairFwdZArgs {
     cx<T>* g3;
     int ny;
     int n;
-    T nu;
     const T* coef;
     int stage;
     const cx<T>* phase;
     const cx<T>* tw;
+    long long ms_u;
+    long long ms_w;
 };
 
 // both members of a pair at one mode: one 16-byte (FP32) / 32-byte access
@@ -97,8 +113,11 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
     const uint32_t kyc = ok ? ky : a.ny - 1;
     constexpr uint32_t CB = sizeof(cx<T>);
     const uint32_t ny = a.ny;
+    // member offsets kept in locals: writing into the argument struct would
+    // copy it to private memory
+    const long long zw = blockIdx.z * a.ms_w;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
-    const Buf<T> U(a.u3 + plane * 3, N * ny * 3 * CB);
+    const Buf<T> U(a.u3 + blockIdx.z * a.ms_u + plane * 3, N * ny * 3 * CB);
     const uint32_t u_lane = ((tid / NL) * ny + kyc) * 3 * CB, u_row = ny * 3 * CB;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
@@ -109,7 +128,7 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
     };
     auto store = [&](cx<T>* dst) {
         if (!ok) return;
-        const Buf<T> b(dst + plane * 2, p_bytes);
+        const Buf<T> b(dst + zw + plane * 2, p_bytes);
 #pragma unroll
         for (int i = 0; i < TLS; ++i)
 #pragma unroll
@@ -187,7 +206,8 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
 // i.e. NL*N consecutive complex values, member f of pair line g is line 2g+f
 // ---------------------------------------------------------------------------
 template <typename T, int N, bool INV>
-__global__ void __launch_bounds__((Tile<T, N>::NT)) k_plines(cx<T>* data, long long npairlines, const cx<T>* tw) {
+__global__ void __launch_bounds__((Tile<T, N>::NT))
+    k_plines(cx<T>* data, long long npairlines, long long member_stride, const cx<T>* tw) {
     using TL = Tile<T, N>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
@@ -195,7 +215,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_plines(cx<T>* data, long l
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const long long g0 = static_cast<long long>(blockIdx.x) * (NL / 2);
-    cx<T>* base = data + g0 * 2 * N;
+    cx<T>* base = data + blockIdx.y * member_stride + g0 * 2 * N;
     const int valid = static_cast<int>(npairlines - g0 < NL / 2 ? npairlines - g0 : NL / 2) * 2 * N;
     load_twiddles<T, N, NT>(twl, tw, tid);
 #pragma unroll
@@ -229,10 +249,11 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * NL;
     const int z = blockIdx.y;
+    const long long zw = blockIdx.z * a.ms_w, zp = blockIdx.z * a.ms_p;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
 #pragma unroll 1
     for (int p = 0; p < a.npairs; ++p) {
-        const cx<T>* AB = pick(a.ab, p);
+        const cx<T>* AB = pick(a.ab, p) + zw;
 #pragma unroll
         for (int i = 0; i < LPT; ++i) {
             const int idx = tid + i * NT;
@@ -255,8 +276,9 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
         }
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
-        T* OA = pick(a.oa, p);
+        T* OA = pick(a.oa, p) + zp;
         T* OB = pick(a.ob, p);
+        if (OB) OB += zp;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int idx = tid + i * NT;
@@ -287,6 +309,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pfwd_x(PairFwdXArgs<T> a) 
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * NL;
     const int z = blockIdx.y;
+    const long long zw = blockIdx.z * a.ms_w, zp = blockIdx.z * a.ms_p;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
     T u[3][E], s[3][E];
 #pragma unroll
@@ -297,8 +320,8 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pfwd_x(PairFwdXArgs<T> a) 
         const int o = (z * a.ny + (ok ? y0 + l : 0)) * N + x;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            u[c][i] = ok ? a.in[c][o] : T(0);
-            s[c][i] = ok ? a.in[3 + c][o] : T(0);
+            u[c][i] = ok ? a.in[c][zp + o] : T(0);
+            s[c][i] = ok ? a.in[3 + c][zp + o] : T(0);
         }
     }
     constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
@@ -313,7 +336,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pfwd_x(PairFwdXArgs<T> a) 
         }
         __syncthreads();
         fft_tile<T, N, NL, NT, false>(buf, twl, tid);
-        cx<T>* out = a.out[p];
+        cx<T>* out = a.out[p] + zw;
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const int idx = tid + i * NT;
@@ -354,6 +377,10 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
     const uint32_t kyc = ok ? ky : a.ny - 1;
     constexpr uint32_t CB = sizeof(cx<T>);
     const uint32_t ny = a.ny;
+    const long long zw = blockIdx.z * a.ms_w;
+    cx<T>* const u3 = a.u3 + blockIdx.z * a.ms_u;
+    cx<T>* const g3 = a.g3 + blockIdx.z * a.ms_u;
+    const T* const coef = a.coef + blockIdx.z * kCoefStride;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
@@ -370,7 +397,7 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
     for (int h = 0; h < HS; ++h) r0[h] = r1[h] = r2[h] = czero<T>();
     cx<T> v[1][T0][R0], w[1][TLS][RL], vn[T0][R0];
     auto load_pair = [&](int p, cx<T> (&dst)[T0][R0]) {
-        const Buf<T> b(pick(a.in, p) + plane * 2, p_bytes);
+        const Buf<T> b(pick(a.in, p) + zw + plane * 2, p_bytes);
 #pragma unroll
         for (int i = 0; i < T0; ++i)
 #pragma unroll
@@ -422,11 +449,12 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
         }
     }
     if (!ok) return;
-    const T alpha = a.coef[2 * a.stage];
-    const T beta = a.coef[2 * a.stage + 1];
+    const T alpha = coef[2 * a.stage];
+    const T beta = coef[2 * a.stage + 1];
+    const T nu = coef[6];
     const bool first = a.stage == 0;
-    const Buf<T> U(a.u3 + plane * 3, N * ny * 3 * CB);
-    const Buf<T> G(a.g3 + plane * 3, N * ny * 3 * CB);
+    const Buf<T> U(u3 + plane * 3, N * ny * 3 * CB);
+    const Buf<T> G(g3 + plane * 3, N * ny * 3 * CB);
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
         const int e = owned_e(h);
@@ -436,7 +464,7 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
         const bool keep = keep_mode(k2, a.n);
         cx<T> u[3] = {U.ld(off), U.ld(off + CB), U.ld(off + 2 * CB)};
         cx<T> rr[3] = {r0[h], r1[h], r2[h]};
-        const T vk2 = a.nu * T(k2);
+        const T vk2 = nu * T(k2);
 #pragma unroll
         for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[c] * vk2;
         project_or_zero(rr[0], rr[1], rr[2], kx, my, mz, k2, keep);

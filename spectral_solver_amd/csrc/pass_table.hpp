@@ -18,11 +18,12 @@ struct PassTable {
     void (*fwd_x)(const FwdXArgs<T>&, int mode, int ny, int nz, hipStream_t);
     void (*fwd_z)(const FwdZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
     // solver stage on pair-interleaved buffers
-    void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
-    void (*plines)(cx<T>* data, long long npairlines, bool inverse, const cx<T>* tw, hipStream_t);
-    void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, hipStream_t);
-    void (*pfwd_x)(const PairFwdXArgs<T>&, int ny, int nz, hipStream_t);
-    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, hipStream_t);
+    void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, int members, hipStream_t);
+    void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
+                   const cx<T>* tw, hipStream_t);
+    void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
+    void (*pfwd_x)(const PairFwdXArgs<T>&, int ny, int nz, int members, hipStream_t);
+    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, int members, hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -72,8 +73,10 @@ struct Launch {
         else
             k_fwd_z<T, N, FZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static void pinv_z(const PairInvZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
-        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx);
+    // pair-buffer passes; `members` equal-size solves per launch (grid z, or
+    // grid y for the y lines)
+    static void pinv_z(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, hipStream_t s) {
+        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
         if (mode == PZ_RHS)
             k_pinv_z<T, N, PZ_RHS><<<grid, ZT::NT, 0, s>>>(a);
         else if (mode == PZ_DIAG)
@@ -81,23 +84,24 @@ struct Launch {
         else
             k_pinv_z<T, N, PZ_VEL><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static void plines(cx<T>* data, long long npairlines, bool inverse, const cx<T>* tw, hipStream_t s) {
-        const dim3 grid(static_cast<unsigned>((npairlines + TL::NL / 2 - 1) / (TL::NL / 2)));
+    static void plines(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
+                       const cx<T>* tw, hipStream_t s) {
+        const dim3 grid(static_cast<unsigned>((npairlines + TL::NL / 2 - 1) / (TL::NL / 2)), members);
         if (inverse)
-            k_plines<T, N, true><<<grid, TL::NT, 0, s>>>(data, npairlines, tw);
+            k_plines<T, N, true><<<grid, TL::NT, 0, s>>>(data, npairlines, member_stride, tw);
         else
-            k_plines<T, N, false><<<grid, TL::NT, 0, s>>>(data, npairlines, tw);
+            k_plines<T, N, false><<<grid, TL::NT, 0, s>>>(data, npairlines, member_stride, tw);
     }
-    static void pinv_x(const PairInvXArgs<T>& a, int ny, int nz, hipStream_t s) {
-        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+    static void pinv_x(const PairInvXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
         k_pinv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
-    static void pfwd_x(const PairFwdXArgs<T>& a, int ny, int nz, hipStream_t s) {
-        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
+    static void pfwd_x(const PairFwdXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
         k_pfwd_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
-    static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, hipStream_t s) {
-        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx);
+    static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, int members, hipStream_t s) {
+        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
         k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
     }
     static constexpr PassTable<T> table{N,     TL::NL,  TL::NT,  &inv_z,  &lines,  &inv_x,    &fwd_x,

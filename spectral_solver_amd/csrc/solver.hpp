@@ -1,0 +1,738 @@
+// Solvers on the pair-buffer pass kernels.
+//
+// Group<T>: `count` independent solves of one grid size n that share every
+// launch (the member index is a grid dimension of each pass kernel). The
+// Taylor-Green solver (TaylorGreenSolver, src/ava_taylor_green_impl.hpp:658-1308)
+// is a group of one; the batched ensemble holds one group per distinct size and
+// captures all groups' stages into one hipGraph per step.
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "spectral_hip.h"
+
+namespace sph {
+
+// ---------------------------------------------------------------------------
+// RK3 coefficient blocks (alpha_s, beta_s*dt for s = 0..2, nu), uploaded
+// through kernel arguments so that no host staging buffer can be overwritten
+// while a copy is pending.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CoefBlock {
+    T v[32 * kCoefStride];
+    int count;
+};
+
+template <typename T>
+__global__ void k_set_coef_block(T* dst, CoefBlock<T> b) {
+    const int i = threadIdx.x;
+    if (i < b.count * kCoefStride) dst[i] = b.v[i];
+}
+
+// Williamson coefficients computed in T exactly as the reference writes them
+// (impl.hpp:998, 1011, 1014, 1027, 1030).
+template <typename T>
+void rk_coefficients(double dt, T nu, T* out) {
+    const T rdt = static_cast<T>(dt);
+    out[0] = T(0);
+    out[1] = rdt / T(3);
+    out[2] = -T(5) / T(9);
+    out[3] = T(15) * rdt / T(16);
+    out[4] = -T(153) / T(128);
+    out[5] = T(8) * rdt / T(15);
+    out[6] = nu;
+    out[7] = T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Seeded random-phase solenoidal field, E(k) ~ k^4 exp(-2 (k/4)^2).
+//
+// Half-spectrum layout [kx][kz][ky], interleaved (re, im) doubles. Every kept
+// mode (|m|^2 <= (n/2)^2, no Nyquist index) gets a random-phase complex
+// 3-vector from SplitMix64 keyed by (seed, signed mode, component), projected
+// perpendicular to k and scaled to |u_hat|^2 ~ E(k) / (4 pi k^2); the kx = 0
+// plane is made Hermitian. The field is finally scaled to kinetic energy
+// <|u|^2>/2 = energy (unnormalised forward transform, so
+// <|u|^2> = sum over the full spectrum of |u_hat|^2 / n^6).
+// ---------------------------------------------------------------------------
+inline std::uint64_t splitmix64(std::uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+inline void random_solenoidal_field(int n, std::uint64_t seed, double energy, double* u0, double* u1, double* u2) {
+    if (n < 4 || n % 2) throw std::runtime_error("random field needs an even grid size >= 4");
+    if (!(energy > 0.0)) throw std::runtime_error("random field energy must be positive");
+    const int H = n / 2 + 1;
+    const std::size_t modes = static_cast<std::size_t>(H) * n * n;
+    double* u[3] = {u0, u1, u2};
+    for (int c = 0; c < 3; ++c) std::fill(u[c], u[c] + 2 * modes, 0.0);
+    auto sgn = [n](int i) { return i <= n / 2 ? i : i - n; };
+    auto slot = [n](int kx, int kz, int ky) { return (static_cast<std::size_t>(kx) * n + kz) * n + ky; };
+    double sum = 0.0;
+    for (int kx = 0; kx < H; ++kx)
+        for (int kz = 0; kz < n; ++kz)
+            for (int ky = 0; ky < n; ++ky) {
+                const int mx = kx, my = sgn(ky), mz = sgn(kz);
+                const long k2 = static_cast<long>(mx) * mx + static_cast<long>(my) * my + static_cast<long>(mz) * mz;
+                if (k2 == 0 || 4 * k2 > static_cast<long>(n) * n) continue;
+                if (mx == n / 2 || my == n / 2 || mz == n / 2) continue;
+                if (kx == 0 && !(my > 0 || (my == 0 && mz > 0))) continue;  // filled from its partner
+                const std::uint64_t key =
+                    (static_cast<std::uint64_t>(mx + n) * (2 * n + 1) + static_cast<std::uint64_t>(my + n)) *
+                        (2 * n + 1) +
+                    static_cast<std::uint64_t>(mz + n);
+                std::complex<double> v[3];
+                for (int c = 0; c < 3; ++c) {
+                    const std::uint64_t r = splitmix64(seed * 0x2545F4914F6CDD1DULL + key * 3 + c);
+                    const double th = 2.0 * kPi * static_cast<double>(r >> 11) * 0x1.0p-53;
+                    v[c] = std::polar(1.0, th);
+                }
+                const double kk[3] = {double(mx), double(my), double(mz)};
+                const std::complex<double> dot = kk[0] * v[0] + kk[1] * v[1] + kk[2] * v[2];
+                double norm = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    v[c] -= kk[c] * dot / static_cast<double>(k2);
+                    norm += std::norm(v[c]);
+                }
+                if (norm < 1e-20) continue;
+                const double k = std::sqrt(static_cast<double>(k2));
+                const double ek = k * k * k * k * std::exp(-2.0 * (k / 4.0) * (k / 4.0));
+                const double amp = std::sqrt(ek / (4.0 * kPi * k * k) / norm);
+                const std::size_t i = slot(kx, kz, ky);
+                for (int c = 0; c < 3; ++c) {
+                    const std::complex<double> w = amp * v[c];
+                    u[c][2 * i] = w.real();
+                    u[c][2 * i + 1] = w.imag();
+                }
+                double e = 0.0;
+                for (int c = 0; c < 3; ++c) e += std::norm(amp * v[c]);
+                if (kx == 0) {
+                    const std::size_t j = slot(0, (n - kz) % n, (n - ky) % n);
+                    for (int c = 0; c < 3; ++c) {
+                        u[c][2 * j] = u[c][2 * i];
+                        u[c][2 * j + 1] = -u[c][2 * i + 1];
+                    }
+                    sum += 2.0 * e;  // the mode and its partner
+                } else {
+                    sum += 2.0 * e;  // the mode and its conjugate at -k (not stored)
+                }
+            }
+    if (!(sum > 0.0)) throw std::runtime_error("random field has no kept modes");
+    const double n6 = std::pow(static_cast<double>(n), 6);
+    const double scale = std::sqrt(energy * n6 / (0.5 * sum));
+    for (int c = 0; c < 3; ++c)
+        for (std::size_t q = 0; q < 2 * modes; ++q) u[c][q] *= scale;
+}
+
+// ---------------------------------------------------------------------------
+// A group of `count` equal-size solves.
+// ---------------------------------------------------------------------------
+template <typename T>
+class Group {
+public:
+    Group(int n_in, int count_in, const std::vector<double>& nu) : n(n_in), count(count_in) {
+        if (count < 1) throw std::runtime_error("a group needs at least one member");
+        if (static_cast<int>(nu.size()) != count) throw std::runtime_error("one viscosity per member");
+        pt_ = table_or_throw<T>(n);
+        H = n / 2 + 1;
+        modes = static_cast<std::int64_t>(H) * n * n;
+        npts = static_cast<std::int64_t>(n) * n * n;
+        su = 3 * modes;
+        sw = 12 * modes;
+        sp = 6 * npts;
+        U_.resize(count * su);
+        G_.resize(count * su);
+        W_.resize(count * sw);
+        P_.resize(count * sp);
+        coef_.resize(static_cast<std::size_t>(count) * kCoefStride);
+        partial_.resize(3 * kReduceBlocks + 8);
+        result_.resize(8);
+        upload(tw_, twiddle_table<T>(n));
+        upload(phase_, phase_table<T>(n));
+        hip_check(hipMemset(G_.get(), 0, count * su * sizeof(cx<T>)), "hipMemset");
+        hip_check(hipMemset(U_.get(), 0, count * su * sizeof(cx<T>)), "hipMemset");
+        // the solver streams are non-blocking: finish the clears first
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        nu_.resize(count);
+        for (int m = 0; m < count; ++m) nu_[m] = static_cast<T>(nu[m]);
+        last_dt_.assign(count, std::nan(""));
+    }
+
+    int n = 0, count = 0, H = 0;
+    std::int64_t modes = 0, npts = 0, su = 0, sw = 0, sp = 0;
+
+    T nu(int m) const { return nu_[m]; }
+    cx<T>* U() const { return U_.get(); }
+    cx<T>* G() const { return G_.get(); }
+
+    // Upload the coefficient blocks of the members whose dt changed.
+    void set_dt(const double* dt, hipStream_t s) {
+        bool changed = false;
+        for (int m = 0; m < count; ++m) changed |= !(dt[m] == last_dt_[m]);
+        if (!changed) return;
+        for (int m0 = 0; m0 < count; m0 += 32) {
+            CoefBlock<T> b{};
+            b.count = std::min(32, count - m0);
+            for (int m = 0; m < b.count; ++m) rk_coefficients<T>(dt[m0 + m], nu_[m0 + m], b.v + m * kCoefStride);
+            k_set_coef_block<T><<<1, 32 * kCoefStride, 0, s>>>(coef_.get() + m0 * kCoefStride, b);
+        }
+        for (int m = 0; m < count; ++m) last_dt_[m] = dt[m];
+        hip_check(hipGetLastError(), "coefficient upload");
+    }
+
+    // pass k (0 inv_z, 1 inv_y, 2 inv_x, 3 fwd_x, 4 fwd_y, 5 fwd_z+update)
+    // of stage `stage` for members [first, first + members); u and g address
+    // member 0 of the state family.
+    void launch_pass(int k, int stage, cx<T>* u, cx<T>* g, int first, int members, hipStream_t s) const {
+        const std::int64_t M2 = 2 * modes;
+        cx<T>* W = W_.get() + first * sw;
+        T* P = P_.get() + first * sp;
+        u += first * su;
+        g += first * su;
+        switch (k) {
+            case 0: {
+                PairInvZArgs<T> a{};
+                a.u3 = u;
+                for (int c = 0; c < 3; ++c) a.out[c] = W + c * M2;
+                a.ny = n;
+                a.n = n;
+                a.phase = phase_.get();
+                a.tw = tw_.get();
+                a.ms_u = su;
+                a.ms_w = sw;
+                pt_->pinv_z(a, PZ_RHS, n, H, members, s);
+                break;
+            }
+            case 1:
+                pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), s);
+                break;
+            case 2: {
+                PairInvXArgs<T> a{};
+                for (int c = 0; c < 3; ++c) {
+                    a.ab[c] = W + c * M2;
+                    a.oa[c] = P + c * npts;
+                    a.ob[c] = P + (3 + c) * npts;
+                }
+                a.npairs = 3;
+                a.ny = n;
+                a.nz = n;
+                a.scale = T(1) / static_cast<T>(npts);
+                a.tw = tw_.get();
+                a.ms_w = sw;
+                a.ms_p = sp;
+                pt_->pinv_x(a, n, n, members, s);
+                break;
+            }
+            case 3: {
+                PairFwdXArgs<T> a{};
+                for (int f = 0; f < 6; ++f) a.in[f] = P + f * npts;
+                for (int q = 0; q < 6; ++q) a.out[q] = W + q * M2;
+                a.ny = n;
+                a.nz = n;
+                a.tw = tw_.get();
+                a.ms_p = sp;
+                a.ms_w = sw;
+                pt_->pfwd_x(a, n, n, members, s);
+                break;
+            }
+            case 4:
+                pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), s);
+                break;
+            default: {
+                PairFwdZArgs<T> a{};
+                for (int q = 0; q < 6; ++q) a.in[q] = W + q * M2;
+                a.u3 = u;
+                a.g3 = g;
+                a.ny = n;
+                a.n = n;
+                a.coef = coef_.get() + first * kCoefStride;
+                a.stage = stage;
+                a.phase = phase_.get();
+                a.tw = tw_.get();
+                a.ms_u = su;
+                a.ms_w = sw;
+                pt_->pfwd_z_rk(a, n, H, members, s);
+                break;
+            }
+        }
+    }
+
+    void launch_stage(int stage, hipStream_t s) const {
+        for (int k = 0; k < 6; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
+    }
+
+    // member m: u_hat -> physical. PZ_DIAG: u in P[0..2], omega in P[3..5];
+    // PZ_VEL: u in P[0..2].
+    void inverse(int m, int mode, hipStream_t s) const {
+        const std::int64_t M2 = 2 * modes;
+        cx<T>* W = W_.get() + m * sw;
+        T* P = P_.get() + m * sp;
+        PairInvZArgs<T> a{};
+        a.u3 = U_.get() + m * su;
+        for (int c = 0; c < 3; ++c) a.out[c] = W + c * M2;
+        a.ny = n;
+        a.n = n;
+        a.phase = phase_.get();
+        a.tw = tw_.get();
+        PairInvXArgs<T> x{};
+        x.ny = n;
+        x.nz = n;
+        x.scale = T(1) / static_cast<T>(npts);
+        x.tw = tw_.get();
+        if (mode == PZ_DIAG) {
+            pt_->pinv_z(a, PZ_DIAG, n, H, 1, s);
+            pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), s);
+            for (int c = 0; c < 3; ++c) {
+                x.ab[c] = W + c * M2;
+                x.oa[c] = P + c * npts;
+                x.ob[c] = P + (3 + c) * npts;
+            }
+            x.npairs = 3;
+        } else {
+            pt_->pinv_z(a, PZ_VEL, n, H, 1, s);
+            pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), s);
+            x.ab[0] = W;
+            x.oa[0] = P;
+            x.ob[0] = P + npts;
+            x.ab[1] = W + M2;
+            x.oa[1] = P + 2 * npts;
+            x.ob[1] = nullptr;
+            x.npairs = 2;
+        }
+        pt_->pinv_x(x, n, n, 1, s);
+        hip_check(hipGetLastError(), "inverse transform launch");
+    }
+
+    // impl.hpp:1122-1179 for member m (reductions in FP64)
+    sph_diagnostics diagnostics(int m, double t, hipStream_t s) {
+        inverse(m, PZ_DIAG, s);
+        T* P = P_.get() + m * sp;
+        k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P, P + npts, P + 2 * npts, P + 3 * npts,
+                                                                   P + 4 * npts, P + 5 * npts, npts, partial_.get());
+        k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
+        k_reduce_div3<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(U_.get() + m * su, n, partial_.get());
+        k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 1, 0b1, result_.get() + 3);
+        double r[4];
+        hip_check(hipMemcpyAsync(r, result_.get(), sizeof(r), hipMemcpyDeviceToHost, s), "hipMemcpy diagnostics");
+        hip_check(hipStreamSynchronize(s), "diagnostics");
+        const double inv = 1.0 / static_cast<double>(npts);
+        sph_diagnostics d{};
+        d.t = t;
+        d.kinetic_energy = r[0] * inv;
+        d.enstrophy = r[2] * inv;
+        d.dissipation = 2.0 * static_cast<double>(nu_[m]) * d.enstrophy;
+        d.max_spectral_divergence = r[3];
+        d.max_velocity = std::sqrt(r[1]);
+        return d;
+    }
+
+    // impl.hpp:1105-1120 for member m
+    double max_velocity(int m, hipStream_t s) {
+        inverse(m, PZ_VEL, s);
+        T* P = P_.get() + m * sp;
+        k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P, P + npts, P + 2 * npts, nullptr, nullptr,
+                                                                   nullptr, npts, partial_.get());
+        k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
+        double r[3];
+        hip_check(hipMemcpyAsync(r, result_.get(), sizeof(r), hipMemcpyDeviceToHost, s), "hipMemcpy max_velocity");
+        hip_check(hipStreamSynchronize(s), "max_velocity");
+        return std::sqrt(r[1]);
+    }
+
+    // component c of member m (component-interleaved state: strided copies)
+    void get_uhat(int m, int c, double* host, hipStream_t s) const {
+        if (c < 0 || c > 2) throw std::runtime_error("component out of range");
+        std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
+        hip_check(hipMemcpy2DAsync(tmp.data(), sizeof(cx<T>), U_.get() + m * su + c, 3 * sizeof(cx<T>),
+                                   sizeof(cx<T>), modes, hipMemcpyDeviceToHost, s),
+                  "hipMemcpy get_uhat");
+        hip_check(hipStreamSynchronize(s), "get_uhat");
+        for (std::int64_t i = 0; i < modes; ++i) {
+            host[2 * i] = tmp[i].x;
+            host[2 * i + 1] = tmp[i].y;
+        }
+    }
+
+    void set_uhat(int m, int c, const double* host, hipStream_t s) {
+        if (c < 0 || c > 2) throw std::runtime_error("component out of range");
+        std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
+        for (std::int64_t i = 0; i < modes; ++i) tmp[i] = {static_cast<T>(host[2 * i]), static_cast<T>(host[2 * i + 1])};
+        hip_check(hipMemcpy2DAsync(U_.get() + m * su + c, 3 * sizeof(cx<T>), tmp.data(), sizeof(cx<T>),
+                                   sizeof(cx<T>), modes, hipMemcpyHostToDevice, s),
+                  "hipMemcpy set_uhat");
+        hip_check(hipStreamSynchronize(s), "set_uhat");
+    }
+
+    // impl.hpp:796-817 for member m: TG field at cell centres, 3 forward
+    // transforms, project_truncate
+    void taylor_green(int m, hipStream_t s) {
+        const std::int64_t M = modes;
+        T* P = P_.get() + m * sp;
+        cx<T>* W = W_.get() + m * sw;
+        k_taylor_green<T><<<2048, 256, 0, s>>>(P, P + npts, P + 2 * npts, n);
+        FwdXArgs<T> fx{};
+        for (int c = 0; c < 3; ++c) {
+            fx.in[c] = P + c * npts;
+            fx.out[c] = W + c * M;
+        }
+        fx.nin = 3;
+        fx.ny = n;
+        fx.nz = n;
+        fx.tw = tw_.get();
+        pt_->fwd_x(fx, FX_PLAIN, n, n, s);
+        pt_->lines(W, 3LL * H * n, false, tw_.get(), s);
+        FwdZArgs<T> fz{};
+        for (int c = 0; c < 3; ++c) fz.in[c] = W + c * M;
+        fz.u3 = U_.get() + m * su;
+        fz.ny = n;
+        fz.n = n;
+        fz.tw = tw_.get();
+        pt_->fwd_z(fz, FZ_PROJECT, n, H, s);
+        hip_check(hipGetLastError(), "initial condition launch");
+    }
+
+    // Mean duration of each pass kernel (all members) over `reps` launches;
+    // the RK pass updates scratch copies of the state so the state is kept.
+    void time_passes(int reps, double* ms, double* bytes, hipStream_t s) {
+        if (reps < 1) reps = 1;
+        hip_check(hipStreamSynchronize(s), "time_passes");
+        DeviceArray<cx<T>> u2(count * su), g2(count * su);
+        hip_check(hipMemcpyAsync(u2.get(), U_.get(), count * su * sizeof(cx<T>), hipMemcpyDeviceToDevice, s), "copy");
+        hip_check(hipMemcpyAsync(g2.get(), G_.get(), count * su * sizeof(cx<T>), hipMemcpyDeviceToDevice, s), "copy");
+        if (std::isnan(last_dt_[0])) {
+            std::vector<double> zero(count, 0.0);
+            set_dt(zero.data(), s);
+        }
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        for (int k = 0; k < 6; ++k) {
+            // warm-up; the passes before k make its inputs genuine
+            for (int j = 0; j <= k; ++j) launch_pass(j, 1, u2.get(), g2.get(), 0, count, s);
+            hip_check(hipEventRecord(a, s), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) launch_pass(k, 1, u2.get(), g2.get(), 0, count, s);
+            hip_check(hipEventRecord(b, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float e = 0.f;
+            hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
+            ms[k] = static_cast<double>(e) / reps;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        hip_check(hipGetLastError(), "time_passes");
+        const double C = static_cast<double>(modes) * 2.0 * sizeof(T) * count;
+        const double R = static_cast<double>(npts) * sizeof(T) * count;
+        const double b6[6] = {9 * C, 12 * C, 6 * C + 6 * R, 6 * R + 12 * C, 24 * C, 24 * C};
+        for (int k = 0; k < 6; ++k) bytes[k] = b6[k];
+        hip_check(hipStreamSynchronize(s), "time_passes");
+    }
+
+private:
+    const PassTable<T>* pt_ = nullptr;
+    DeviceArray<cx<T>> U_, G_, W_, tw_, phase_;
+    DeviceArray<T> P_, coef_;
+    DeviceArray<double> partial_, result_;
+    std::vector<T> nu_;
+    std::vector<double> last_dt_;
+};
+
+// Per-step GPU time from event pairs, resolved lazily.
+class StepTimer {
+public:
+    ~StepTimer() {
+        for (auto& p : pending_) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+    std::pair<hipEvent_t, hipEvent_t> begin(hipStream_t s) {
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+        return {e0, e1};
+    }
+    void end(std::pair<hipEvent_t, hipEvent_t> p, hipStream_t s) {
+        hip_check(hipEventRecord(p.second, s), "hipEventRecord");
+        pending_.push_back(p);
+        ++steps;
+        if (pending_.size() > 64) resolve_one();
+    }
+    double seconds() {
+        while (!pending_.empty()) resolve_one();
+        return total_;
+    }
+    long long steps = 0;
+
+private:
+    void resolve_one() {
+        auto p = pending_.front();
+        pending_.pop_front();
+        hip_check(hipEventSynchronize(p.second), "hipEventSynchronize");
+        float e = 0.f;
+        hip_check(hipEventElapsedTime(&e, p.first, p.second), "hipEventElapsedTime");
+        total_ += 1e-3 * e;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    std::deque<std::pair<hipEvent_t, hipEvent_t>> pending_;
+    double total_ = 0.0;
+};
+
+// A captured step: every stage of every group, in order.
+class StepGraph {
+public:
+    ~StepGraph() { reset(); }
+    template <class F>
+    void launch(hipStream_t s, F&& record) {
+        if (!exec_) {
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            record();
+            hip_check(hipStreamEndCapture(s, &graph_), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+        }
+        hip_check(hipGraphLaunch(exec_, s), "hipGraphLaunch");
+    }
+    void reset() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+private:
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Taylor-Green solver (one member)
+// ---------------------------------------------------------------------------
+class SolverBase {
+public:
+    virtual ~SolverBase() = default;
+    virtual void step(double dt) = 0;
+    virtual sph_diagnostics diagnostics(double t) = 0;
+    virtual double max_velocity() = 0;
+    virtual void get_uhat(int c, double* host) = 0;
+    virtual void set_uhat(int c, const double* host) = 0;
+    virtual void profile(sph_profile* out) = 0;
+    virtual void set_graph(bool on) = 0;
+    virtual void synchronize() = 0;
+    virtual void time_passes(int reps, double* ms, double* bytes) = 0;
+    int n = 0;
+    std::int64_t modes = 0;
+};
+
+template <typename T>
+class Solver final : public SolverBase {
+public:
+    Solver(int n_in, double reynolds) : g_(n_in, 1, {1.0 / reynolds}) {
+        n = n_in;
+        modes = g_.modes;
+        g_.taylor_green(0, stream_.get());
+        prof_.forward_calls += 3;
+        hip_check(hipStreamSynchronize(stream_.get()), "TaylorGreenSolver initialisation");
+    }
+    ~Solver() override { (void)hipStreamSynchronize(stream_.get()); }
+
+    void step(double dt) override {
+        const hipStream_t s = stream_.get();
+        g_.set_dt(&dt, s);
+        auto ev = timer_.begin(s);
+        if (use_graph_) {
+            graph_.launch(s, [&] {
+                for (int st = 0; st < 3; ++st) g_.launch_stage(st, s);
+            });
+        } else {
+            for (int st = 0; st < 3; ++st) g_.launch_stage(st, s);
+            hip_check(hipGetLastError(), "TaylorGreenSolver::step");
+        }
+        timer_.end(ev, s);
+        prof_.step_calls += 1;
+        prof_.rhs_calls += 3;
+        prof_.forward_calls += 36;
+        prof_.backward_calls += 18;
+    }
+
+    sph_diagnostics diagnostics(double t) override {
+        const auto t0 = std::chrono::steady_clock::now();
+        const sph_diagnostics d = g_.diagnostics(0, t, stream_.get());
+        prof_.diagnostics += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        prof_.diagnostics_calls += 1;
+        prof_.backward_calls += 6;
+        return d;
+    }
+
+    double max_velocity() override {
+        const auto t0 = std::chrono::steady_clock::now();
+        const double v = g_.max_velocity(0, stream_.get());
+        prof_.max_velocity += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        prof_.max_velocity_calls += 1;
+        prof_.backward_calls += 3;
+        return v;
+    }
+
+    void get_uhat(int c, double* host) override { g_.get_uhat(0, c, host, stream_.get()); }
+    void set_uhat(int c, const double* host) override { g_.set_uhat(0, c, host, stream_.get()); }
+
+    void profile(sph_profile* out) override {
+        prof_.step = timer_.seconds();
+        *out = prof_;
+    }
+    void set_graph(bool on) override { use_graph_ = on; }
+    void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }
+    void time_passes(int reps, double* ms, double* bytes) override { g_.time_passes(reps, ms, bytes, stream_.get()); }
+
+private:
+    Stream stream_;
+    Group<T> g_;
+    bool use_graph_ = true;
+    StepGraph graph_;
+    StepTimer timer_;
+    sph_profile prof_{};
+};
+
+// ---------------------------------------------------------------------------
+// Batched ensemble of independent solves of uneven sizes
+// ---------------------------------------------------------------------------
+class EnsembleBase {
+public:
+    virtual ~EnsembleBase() = default;
+    virtual void step(const double* dt) = 0;
+    virtual sph_diagnostics diagnostics(int member, double t) = 0;
+    virtual double max_velocity(int member) = 0;
+    virtual void get_uhat(int member, int c, double* host) = 0;
+    virtual void set_uhat(int member, int c, const double* host) = 0;
+    virtual void set_graph(bool on) = 0;
+    virtual void synchronize() = 0;
+    virtual double step_seconds(long long* steps) = 0;
+    virtual void time_passes(int reps, double* ms, double* bytes) = 0;
+    std::vector<int> sizes;
+};
+
+template <typename T>
+class Ensemble final : public EnsembleBase {
+public:
+    Ensemble(int count, const int* n, const double* nu, const std::uint64_t* seeds, double energy) {
+        if (count < 1) throw std::runtime_error("ensemble needs at least one member");
+        sizes.assign(n, n + count);
+        // one group per distinct size, members in order of appearance
+        for (int j = 0; j < count; ++j) {
+            if (n[j] <= 0) throw std::runtime_error("grid size must be positive");
+            if (!(nu[j] > 0.0)) throw std::runtime_error("viscosity must be positive");
+            if (!pass_table<T>(n[j])) throw std::runtime_error("unsupported transform length " + std::to_string(n[j]));
+            int gi = -1;
+            for (std::size_t q = 0; q < group_n_.size(); ++q)
+                if (group_n_[q] == n[j]) gi = static_cast<int>(q);
+            if (gi < 0) {
+                gi = static_cast<int>(group_n_.size());
+                group_n_.push_back(n[j]);
+                group_nu_.emplace_back();
+            }
+            where_.push_back({gi, static_cast<int>(group_nu_[gi].size())});
+            group_nu_[gi].push_back(nu[j]);
+        }
+        for (std::size_t q = 0; q < group_n_.size(); ++q)
+            groups_.push_back(std::make_unique<Group<T>>(group_n_[q], static_cast<int>(group_nu_[q].size()),
+                                                         group_nu_[q]));
+        for (int j = 0; j < count; ++j) {
+            const int nj = n[j];
+            const std::size_t modes = static_cast<std::size_t>(nj / 2 + 1) * nj * nj;
+            std::vector<double> u0(2 * modes), u1(2 * modes), u2(2 * modes);
+            random_solenoidal_field(nj, seeds[j], energy, u0.data(), u1.data(), u2.data());
+            set_uhat(j, 0, u0.data());
+            set_uhat(j, 1, u1.data());
+            set_uhat(j, 2, u2.data());
+        }
+        dt_.resize(groups_.size());
+        for (std::size_t q = 0; q < groups_.size(); ++q) dt_[q].assign(groups_[q]->count, 0.0);
+    }
+
+    void step(const double* dt) override {
+        const hipStream_t s = stream_.get();
+        for (std::size_t j = 0; j < where_.size(); ++j) dt_[where_[j].first][where_[j].second] = dt[j];
+        for (std::size_t q = 0; q < groups_.size(); ++q) groups_[q]->set_dt(dt_[q].data(), s);
+        auto ev = timer_.begin(s);
+        auto record = [&] {
+            for (int st = 0; st < 3; ++st)
+                for (auto& g : groups_) g->launch_stage(st, s);
+        };
+        if (use_graph_) {
+            graph_.launch(s, record);
+        } else {
+            record();
+            hip_check(hipGetLastError(), "ensemble step");
+        }
+        timer_.end(ev, s);
+    }
+
+    sph_diagnostics diagnostics(int member, double t) override {
+        const auto w = locate(member);
+        return groups_[w.first]->diagnostics(w.second, t, stream_.get());
+    }
+    double max_velocity(int member) override {
+        const auto w = locate(member);
+        return groups_[w.first]->max_velocity(w.second, stream_.get());
+    }
+    void get_uhat(int member, int c, double* host) override {
+        const auto w = locate(member);
+        groups_[w.first]->get_uhat(w.second, c, host, stream_.get());
+    }
+    void set_uhat(int member, int c, const double* host) override {
+        const auto w = locate(member);
+        groups_[w.first]->set_uhat(w.second, c, host, stream_.get());
+    }
+    void set_graph(bool on) override { use_graph_ = on; }
+    void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }
+    double step_seconds(long long* steps) override {
+        if (steps) *steps = timer_.steps;
+        return timer_.seconds();
+    }
+    // Per pass, summed over the groups (the ensemble's six launches per stage
+    // per group).
+    void time_passes(int reps, double* ms, double* bytes) override {
+        for (int k = 0; k < 6; ++k) ms[k] = bytes[k] = 0.0;
+        for (auto& g : groups_) {
+            double m6[6], b6[6];
+            g->time_passes(reps, m6, b6, stream_.get());
+            for (int k = 0; k < 6; ++k) {
+                ms[k] += m6[k];
+                bytes[k] += b6[k];
+            }
+        }
+    }
+
+private:
+    std::pair<int, int> locate(int member) const {
+        if (member < 0 || member >= static_cast<int>(where_.size())) throw std::runtime_error("member out of range");
+        return where_[member];
+    }
+
+    Stream stream_;
+    std::vector<int> group_n_;
+    std::vector<std::vector<double>> group_nu_;
+    std::vector<std::unique_ptr<Group<T>>> groups_;
+    std::vector<std::pair<int, int>> where_;  // member -> (group, index in group)
+    std::vector<std::vector<double>> dt_;
+    bool use_graph_ = true;
+    StepGraph graph_;
+    StepTimer timer_;
+};
+
+}  // namespace sph

@@ -14,7 +14,8 @@ import spectral_solver_amd as sph
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 16), (12, 12, 12), (32, 16, 8), (8, 24, 48), (64, 64, 64), (96, 32, 16), (128, 8, 12)]
+SHAPES = [(16, 16, 16), (12, 12, 12), (32, 16, 8), (8, 24, 48), (64, 64, 64), (96, 32, 16), (128, 8, 12),
+          (16, 8, 96), (24, 12, 24)]
 
 
 def half_of(c2c):
