@@ -26,8 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PASS_NAMES = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
-KERNEL_NAMES = ("k_pinv_z", "k_plines", "k_pinv_x", "k_pfwd_x", "k_plines", "k_pfwd_z_rk")
+PASS_NAMES = ("inverse_z", "inverse_y", "x_products", "forward_y", "forward_z_update")
+KERNEL_NAMES = ("k_pinv_z", "k_plines", "k_px_products", "k_plines", "k_pfwd_z_rk")
 ENSEMBLE_SIZES = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
 
 
@@ -36,10 +36,11 @@ def log(*a):
 
 
 def step_bytes(n: int, s: int) -> float:
-    """Algorithmic HBM bytes of one RK3 step: 36 R + 261 C."""
-    R = n ** 3 * s
+    """Algorithmic HBM bytes of one RK3 step, 3 x 87 C: per stage inverse z
+    9C, inverse y 12C, x pass 18C, forward y 24C, forward z + update 24C,
+    C = n^2 (n/2+1) complex values of s-byte parts."""
     C = n * n * (n // 2 + 1) * 2 * s
-    return 36.0 * R + 261.0 * C
+    return 261.0 * C
 
 
 def time_steps(sph, n, precision, steps, warmup, dt):
@@ -105,6 +106,40 @@ def cpu_baseline(n: int, dt: float):
                       f"scaled x3 to a step"}
 
 
+def accuracy_run(sph, n: int, dt: float, t_end: float, every: float):
+    """FP32 form against plain FP64 on the same GPU over a whole run: both
+    solvers advance in lock step from t = 0 to t_end with fixed dt, and E, the
+    dissipation and the enstrophy of both are recorded every `every`."""
+    s32 = sph.TaylorGreenSolver(n, 1600.0, "fp32")
+    s64 = sph.TaylorGreenSolver(n, 1600.0, "fp64")
+    steps = int(round(t_end / dt))
+    per = max(1, int(round(every / dt)))
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(steps + 1):
+        if k % per == 0 or k == steps:
+            d32, d64 = s32.diagnostics(k * dt), s64.diagnostics(k * dt)
+            rows.append([k * dt, d64.kinetic_energy, d64.dissipation, d64.enstrophy,
+                         d32.kinetic_energy, d32.dissipation, d32.enstrophy])
+            log(f"t={k * dt:.4f} E64={d64.kinetic_energy:.12f} eps64={d64.dissipation:.10e} "
+                f"eps32 rel err={abs(d32.dissipation - d64.dissipation) / d64.dissipation:.3e} "
+                f"{time.perf_counter() - t0:.0f} s")
+        if k < steps:
+            s32.step_rk3(dt)
+            s64.step_rk3(dt)
+    r = [list(map(float, x)) for x in rows]
+    rel = lambda a, b: abs(a - b) / abs(b)
+    out = {"workload": f"Taylor-Green {n}^3, nu=1/1600, dt={dt}, t=0..{t_end}, FP32 form vs plain FP64",
+           "steps": steps, "seconds": round(time.perf_counter() - t0, 1),
+           "max_rel_err": {"kinetic_energy": max(rel(x[4], x[1]) for x in r),
+                           "dissipation": max(rel(x[5], x[2]) for x in r),
+                           "enstrophy": max(rel(x[6], x[3]) for x in r)},
+           "columns": ["t", "E64", "eps64", "Z64", "E32", "eps32", "Z32"], "rows": r}
+    s32.close()
+    s64.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +152,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--accuracy-to", type=float, default=None, metavar="T",
+                    help="instead of timing: FP32 vs FP64 run to t=T, diagnostics every --accuracy-every")
+    ap.add_argument("--accuracy-every", type=float, default=0.1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the workload is one {args.n}^3 grid per GPU and this "
@@ -125,6 +163,9 @@ def main():
 
     import spectral_solver_amd as sph
     n, K, W = args.n, args.steps, args.warmup
+    if args.accuracy_to is not None:
+        print(json.dumps(accuracy_run(sph, n, args.dt, args.accuracy_to, args.accuracy_every)), flush=True)
+        return
 
     log(f"fp32 {n}^3: warmup {W}, timing {K} steps")
     s32, sec32 = time_steps(sph, n, "fp32", K, W, args.dt)
@@ -146,7 +187,7 @@ def main():
                 "roofline_frac": round(step_bytes(n, 8) / sec64 / 1e9 / HBM_PEAK_GBS, 4)}
         s64.close()
 
-    dom = max(range(6), key=lambda k: passes[PASS_NAMES[k]][0])
+    dom = max(range(len(PASS_NAMES)), key=lambda k: passes[PASS_NAMES[k]][0])
     ms, by = passes[PASS_NAMES[dom]]
     achieved = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

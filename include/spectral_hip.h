@@ -151,12 +151,12 @@ int sph_solver_set_graph(sph_solver_t solver, int enabled);
 int sph_solver_synchronize(sph_solver_t solver);
 
 /* Measure the pass kernels of one RK3 stage with HIP events on the solver's
- * stream: each of the six passes is launched `reps` times back to back on the
- * current state (the RK-update pass in a read-only timing form that writes a
- * scratch copy) and its mean duration in milliseconds is written to ms[6] in
- * the order inverse-z, inverse-y, inverse-x, forward-x, forward-y,
+ * stream: each of the five passes is launched `reps` times back to back on
+ * the current state (the RK-update pass on a scratch copy of the state) and
+ * its mean duration in milliseconds is written to ms[5] in the order
+ * inverse-z, inverse-y, x (inverse x + products + forward x), forward-y,
  * forward-z(+update). Also returns the algorithmic HBM bytes per launch of
- * each pass in bytes[6]. The state is unchanged. */
+ * each pass in bytes[5]. The state is unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
 /* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,

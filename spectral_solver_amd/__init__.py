@@ -353,8 +353,8 @@ class TaylorGreenSolver:
         return {f[0]: getattr(p, f[0]) for f in _Profile._fields_}
 
     def time_passes(self, reps: int = 5):
-        ms = np.zeros(6)
-        by = np.zeros(6)
+        ms = np.zeros(len(_PASS_NAMES))
+        by = np.zeros(len(_PASS_NAMES))
         _check(load_library().sph_solver_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 
@@ -373,7 +373,7 @@ class TaylorGreenSolver:
         _check(load_library().sph_solver_set_uhat(self._h, int(c), _dptr(a.view(np.float64))))
 
 
-_PASS_NAMES = ("inverse_z", "inverse_y", "inverse_x", "forward_x", "forward_y", "forward_z_update")
+_PASS_NAMES = ("inverse_z", "inverse_y", "x_products", "forward_y", "forward_z_update")
 
 
 def random_solenoidal_field(n: int, seed: int, energy: float = 0.5):
@@ -474,8 +474,8 @@ class Ensemble:
         return s.value, k.value
 
     def time_passes(self, reps: int = 5):
-        ms = np.zeros(6)
-        by = np.zeros(6)
+        ms = np.zeros(len(_PASS_NAMES))
+        by = np.zeros(len(_PASS_NAMES))
         _check(load_library().sph_ensemble_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 

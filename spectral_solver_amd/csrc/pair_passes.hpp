@@ -53,17 +53,6 @@ struct PairInvXArgs {
     long long ms_p;
 };
 
-template <typename T>
-struct PairFwdXArgs {
-    const T* in[6];  // u0 u1 u2 s0 s1 s2 (physical)
-    cx<T>* out[6];   // pair fields (P_ab, Q_ab), ab in (00,01,02,11,12,22)
-    int ny;
-    int nz;
-    const cx<T>* tw;
-    long long ms_p;
-    long long ms_w;
-};
-
 // coef: per member 8 values in device memory: alpha_s, beta_s*dt for the
 // three stages, then nu.
 constexpr int kCoefStride = 8;
@@ -235,15 +224,94 @@ __global__ void __launch_bounds__((Tile<T, N>::NT))
 }
 
 // ---------------------------------------------------------------------------
+// x-pass building blocks. A tile is NL x lines (y0 .. y0+NL-1, fixed z) in LDS,
+// element x of line l at buf[x*S + l]; thread tid handles element
+// idx = tid + i*NT (x = idx % N, l = idx / N) of the physical lines.
+// ---------------------------------------------------------------------------
+
+// Spectral x lines of pair field AB -> buf as Z = A + iB over all N modes
+// (Hermitian extension of both members; Re rule on kx = 0 and N/2).
+template <typename T, int N>
+__device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, int tid, int y0, int z, int ny,
+                                                  int nz) {
+    using TL = Tile<T, N>;
+    constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
+    constexpr int LOADS = NL * (N / 2 + 1);
+    constexpr int LPT = (LOADS + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+        const int idx = tid + i * NT;
+        if (LOADS % NT == 0 || idx < LOADS) {
+            const int l = idx % NL, k = idx / NL;
+            cx<T> va = czero<T>(), vb = czero<T>();
+            if (y0 + l < ny) {
+                const size_t off = (static_cast<size_t>(k) * nz + z) * ny + y0 + l;
+                const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
+                va = q.a;
+                vb = q.b;
+            }
+            if (k == 0 || k == N / 2) {
+                va.y = T(0);
+                vb.y = T(0);
+            }
+            buf[k * S + l] = {va.x - vb.y, va.y + vb.x};
+            if (k > 0 && k < N / 2) buf[(N - k) * S + l] = {va.x + vb.y, vb.x - va.y};
+        }
+    }
+}
+
+// The six products (u_a u_b, s_a s_b), ab in (00,01,02,11,12,22), of the
+// physical values held in registers: forward x transform of each pair and
+// separation into the pair field out[p] (P_hat, Q_hat) at the tile's modes.
+template <typename T, int N, int E>
+__device__ __forceinline__ void products_forward_x(const T (&u)[3][E], const T (&s)[3][E], cx<T>* buf,
+                                                   const cx<T>* twl, cx<T>* const (&out)[6], long long zw, int tid,
+                                                   int y0, int z, int ny, int nz) {
+    using TL = Tile<T, N>;
+    constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
+    static_assert(E == TL::E, "register slots must match the tile");
+    constexpr int ITEMS = NL * (N / 2 + 1);
+    constexpr int IPT = (ITEMS + NT - 1) / NT;
+    constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int idx = tid + i * NT;
+            if (TL::ELEMS % NT == 0 || idx < TL::ELEMS)
+                buf[(idx % N) * S + idx / N] = {u[PA[p]][i] * u[PB[p]][i], s[PA[p]][i] * s[PB[p]][i]};
+        }
+        __syncthreads();
+        fft_tile<T, N, NL, NT, false>(buf, twl, tid);
+        cx<T>* o = out[p] + zw;
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const int idx = tid + i * NT;
+            if (ITEMS % NT == 0 || idx < ITEMS) {
+                const int l = idx % NL, k = idx / NL;
+                if (y0 + l < ny) {
+                    const cx<T> zk = buf[k * S + l];
+                    const cx<T> zm = buf[((N - k) % N) * S + l];
+                    const size_t off = (static_cast<size_t>(k) * nz + z) * ny + y0 + l;
+                    cx2<T> q;
+                    q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+                    q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+                    reinterpret_cast<cx2<T>*>(o)[off] = q;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // inverse x (complex-to-real) of pair fields: Z = A + iB per row
 // ---------------------------------------------------------------------------
 template <typename T, int N>
 __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) {
     using TL = Tile<T, N>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
-    constexpr int H = N / 2 + 1;
-    constexpr int LOADS = NL * H;
-    constexpr int LPT = (LOADS + NT - 1) / NT;
     __shared__ cx<T> buf[TL::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
@@ -253,27 +321,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
     load_twiddles<T, N, NT>(twl, a.tw, tid);
 #pragma unroll 1
     for (int p = 0; p < a.npairs; ++p) {
-        const cx<T>* AB = pick(a.ab, p) + zw;
-#pragma unroll
-        for (int i = 0; i < LPT; ++i) {
-            const int idx = tid + i * NT;
-            if (LOADS % NT == 0 || idx < LOADS) {
-                const int l = idx % NL, k = idx / NL;
-                cx<T> va = czero<T>(), vb = czero<T>();
-                if (y0 + l < a.ny) {
-                    const size_t off = (static_cast<size_t>(k) * a.nz + z) * a.ny + y0 + l;
-                    const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
-                    va = q.a;
-                    vb = q.b;
-                }
-                if (k == 0 || k == N / 2) {
-                    va.y = T(0);
-                    vb.y = T(0);
-                }
-                buf[k * S + l] = {va.x - vb.y, va.y + vb.x};
-                if (k > 0 && k < N / 2) buf[(N - k) * S + l] = {va.x + vb.y, vb.x - va.y};
-            }
-        }
+        load_pair_x_lines<T, N>(buf, pick(a.ab, p) + zw, tid, y0, z, a.ny, a.nz);
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
         T* OA = pick(a.oa, p) + zp;
@@ -295,66 +343,54 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
 }
 
 // ---------------------------------------------------------------------------
-// forward x (real-to-complex) of the six product pairs
+// the x pass of the RHS in one kernel: inverse x of the three velocity pairs
+// (u_c, s_c) (the arithmetic of k_pinv_x, scale included), the six products
+// in registers, forward x of the product pairs. The physical fields never
+// touch HBM (impl.hpp:892-911 forms them in full arrays). The product pairs
+// may overwrite the velocity pairs (out[c] == ab[c]): a work-group reads all
+// of its lines before it writes any, and work-groups own disjoint (y, z)
+// lines.
 // ---------------------------------------------------------------------------
+template <typename T>
+struct PairXArgs {
+    const cx<T>* ab[3];  // (u_hat_c, (u_hat phi)_c) after the z and y transforms, [kx][z][y][2]
+    cx<T>* out[6];       // product pairs, [kx][z][y][2]
+    int ny;
+    int nz;
+    T scale;
+    const cx<T>* tw;
+    long long ms_w;
+};
+
 template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N>::NT)) k_pfwd_x(PairFwdXArgs<T> a) {
+__global__ void __launch_bounds__((Tile<T, N>::NT)) k_px_products(PairXArgs<T> a) {
     using TL = Tile<T, N>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
-    constexpr int H = N / 2 + 1;
-    constexpr int ITEMS = NL * H;
-    constexpr int IPT = (ITEMS + NT - 1) / NT;
     __shared__ cx<T> buf[TL::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * NL;
     const int z = blockIdx.y;
-    const long long zw = blockIdx.z * a.ms_w, zp = blockIdx.z * a.ms_p;
+    const long long zw = blockIdx.z * a.ms_w;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
     T u[3][E], s[3][E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int idx = tid + i * NT;
-        const int x = idx % N, l = idx / N;
-        const bool ok = idx < TL::ELEMS && y0 + l < a.ny;
-        const int o = (z * a.ny + (ok ? y0 + l : 0)) * N + x;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            u[c][i] = ok ? a.in[c][zp + o] : T(0);
-            s[c][i] = ok ? a.in[3 + c][zp + o] : T(0);
-        }
-    }
-    constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
-    constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
+    for (int c = 0; c < 3; ++c) {
+        load_pair_x_lines<T, N>(buf, a.ab[c] + zw, tid, y0, z, a.ny, a.nz);
+        __syncthreads();
+        fft_tile<T, N, NL, NT, true>(buf, twl, tid);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int idx = tid + i * NT;
-            if (TL::ELEMS % NT == 0 || idx < TL::ELEMS)
-                buf[(idx % N) * S + idx / N] = {u[PA[p]][i] * u[PB[p]][i], s[PA[p]][i] * s[PB[p]][i]};
-        }
-        __syncthreads();
-        fft_tile<T, N, NL, NT, false>(buf, twl, tid);
-        cx<T>* out = a.out[p] + zw;
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            const int idx = tid + i * NT;
-            if (ITEMS % NT == 0 || idx < ITEMS) {
-                const int l = idx % NL, k = idx / NL;
-                if (y0 + l < a.ny) {
-                    const cx<T> zk = buf[k * S + l];
-                    const cx<T> zm = buf[((N - k) % N) * S + l];
-                    const size_t off = (static_cast<size_t>(k) * a.nz + z) * a.ny + y0 + l;
-                    cx2<T> q;
-                    q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-                    q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                    reinterpret_cast<cx2<T>*>(out)[off] = q;
-                }
-            }
+            const int x = idx % N, l = idx / N;
+            cx<T> v = czero<T>();
+            if ((TL::ELEMS % NT == 0 || idx < TL::ELEMS) && y0 + l < a.ny) v = buf[x * S + l];
+            u[c][i] = v.x * a.scale;
+            s[c][i] = v.y * a.scale;
         }
         __syncthreads();
     }
+    products_forward_x<T, N, E>(u, s, buf, twl, a.out, zw, tid, y0, z, a.ny, a.nz);
 }
 
 // ---------------------------------------------------------------------------

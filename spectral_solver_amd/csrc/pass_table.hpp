@@ -22,7 +22,7 @@ struct PassTable {
     void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                    const cx<T>* tw, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
-    void (*pfwd_x)(const PairFwdXArgs<T>&, int ny, int nz, int members, hipStream_t);
+    void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, hipStream_t);
     void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, int members, hipStream_t);
 };
 
@@ -96,16 +96,16 @@ struct Launch {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
         k_pinv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
-    static void pfwd_x(const PairFwdXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+    static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
-        k_pfwd_x<T, N><<<grid, TL::NT, 0, s>>>(a);
+        k_px_products<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
     static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, int members, hipStream_t s) {
         const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
         k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
     }
     static constexpr PassTable<T> table{N,     TL::NL,  TL::NT,  &inv_z,  &lines,  &inv_x,    &fwd_x,
-                                        &fwd_z, &pinv_z, &plines, &pinv_x, &pfwd_x, &pfwd_z_rk};
+                                        &fwd_z, &pinv_z, &plines, &pinv_x, &px_products, &pfwd_z_rk};
 };
 
 }  // namespace sph

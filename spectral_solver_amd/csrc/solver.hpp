@@ -154,11 +154,12 @@ public:
         npts = static_cast<std::int64_t>(n) * n * n;
         su = 3 * modes;
         sw = 12 * modes;
-        sp = 6 * npts;
         U_.resize(count * su);
         G_.resize(count * su);
         W_.resize(count * sw);
-        P_.resize(count * sp);
+        // physical fields only for diagnostics and the initial condition,
+        // one member at a time
+        P_.resize(6 * npts);
         coef_.resize(static_cast<std::size_t>(count) * kCoefStride);
         partial_.resize(3 * kReduceBlocks + 8);
         result_.resize(8);
@@ -173,8 +174,9 @@ public:
         last_dt_.assign(count, std::nan(""));
     }
 
+    static constexpr int kPasses = 5;  // kernel launches per RK stage
     int n = 0, count = 0, H = 0;
-    std::int64_t modes = 0, npts = 0, su = 0, sw = 0, sp = 0;
+    std::int64_t modes = 0, npts = 0, su = 0, sw = 0;  // per-member strides: state, work
 
     T nu(int m) const { return nu_[m]; }
     cx<T>* U() const { return U_.get(); }
@@ -195,13 +197,12 @@ public:
         hip_check(hipGetLastError(), "coefficient upload");
     }
 
-    // pass k (0 inv_z, 1 inv_y, 2 inv_x, 3 fwd_x, 4 fwd_y, 5 fwd_z+update)
-    // of stage `stage` for members [first, first + members); u and g address
-    // member 0 of the state family.
+    // pass k (0 inverse z, 1 inverse y, 2 inverse x + products + forward x,
+    // 3 forward y, 4 forward z + update) of stage `stage` for members
+    // [first, first + members); u and g address member 0 of the state family.
     void launch_pass(int k, int stage, cx<T>* u, cx<T>* g, int first, int members, hipStream_t s) const {
         const std::int64_t M2 = 2 * modes;
         cx<T>* W = W_.get() + first * sw;
-        T* P = P_.get() + first * sp;
         u += first * su;
         g += first * su;
         switch (k) {
@@ -222,35 +223,18 @@ public:
                 pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), s);
                 break;
             case 2: {
-                PairInvXArgs<T> a{};
-                for (int c = 0; c < 3; ++c) {
-                    a.ab[c] = W + c * M2;
-                    a.oa[c] = P + c * npts;
-                    a.ob[c] = P + (3 + c) * npts;
-                }
-                a.npairs = 3;
+                PairXArgs<T> a{};
+                for (int c = 0; c < 3; ++c) a.ab[c] = W + c * M2;
+                for (int q = 0; q < 6; ++q) a.out[q] = W + q * M2;
                 a.ny = n;
                 a.nz = n;
                 a.scale = T(1) / static_cast<T>(npts);
                 a.tw = tw_.get();
                 a.ms_w = sw;
-                a.ms_p = sp;
-                pt_->pinv_x(a, n, n, members, s);
+                pt_->px_products(a, n, n, members, s);
                 break;
             }
-            case 3: {
-                PairFwdXArgs<T> a{};
-                for (int f = 0; f < 6; ++f) a.in[f] = P + f * npts;
-                for (int q = 0; q < 6; ++q) a.out[q] = W + q * M2;
-                a.ny = n;
-                a.nz = n;
-                a.tw = tw_.get();
-                a.ms_p = sp;
-                a.ms_w = sw;
-                pt_->pfwd_x(a, n, n, members, s);
-                break;
-            }
-            case 4:
+            case 3:
                 pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), s);
                 break;
             default: {
@@ -273,15 +257,15 @@ public:
     }
 
     void launch_stage(int stage, hipStream_t s) const {
-        for (int k = 0; k < 6; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
+        for (int k = 0; k < kPasses; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
     }
 
-    // member m: u_hat -> physical. PZ_DIAG: u in P[0..2], omega in P[3..5];
-    // PZ_VEL: u in P[0..2].
+    // member m: u_hat -> physical (the group's one physical buffer).
+    // PZ_DIAG: u in P[0..2], omega in P[3..5]; PZ_VEL: u in P[0..2].
     void inverse(int m, int mode, hipStream_t s) const {
         const std::int64_t M2 = 2 * modes;
         cx<T>* W = W_.get() + m * sw;
-        T* P = P_.get() + m * sp;
+        T* P = P_.get();
         PairInvZArgs<T> a{};
         a.u3 = U_.get() + m * su;
         for (int c = 0; c < 3; ++c) a.out[c] = W + c * M2;
@@ -321,7 +305,7 @@ public:
     // impl.hpp:1122-1179 for member m (reductions in FP64)
     sph_diagnostics diagnostics(int m, double t, hipStream_t s) {
         inverse(m, PZ_DIAG, s);
-        T* P = P_.get() + m * sp;
+        T* P = P_.get();
         k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P, P + npts, P + 2 * npts, P + 3 * npts,
                                                                    P + 4 * npts, P + 5 * npts, npts, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
@@ -344,7 +328,7 @@ public:
     // impl.hpp:1105-1120 for member m
     double max_velocity(int m, hipStream_t s) {
         inverse(m, PZ_VEL, s);
-        T* P = P_.get() + m * sp;
+        T* P = P_.get();
         k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P, P + npts, P + 2 * npts, nullptr, nullptr,
                                                                    nullptr, npts, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
@@ -355,34 +339,44 @@ public:
     }
 
     // component c of member m (component-interleaved state: strided copies)
+    // Component c of member m. The state is component-interleaved, so the
+    // member's whole state moves in one contiguous copy (a strided copy of
+    // 8-byte rows is orders of magnitude slower) and c is picked on the host.
     void get_uhat(int m, int c, double* host, hipStream_t s) const {
         if (c < 0 || c > 2) throw std::runtime_error("component out of range");
-        std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
-        hip_check(hipMemcpy2DAsync(tmp.data(), sizeof(cx<T>), U_.get() + m * su + c, 3 * sizeof(cx<T>),
-                                   sizeof(cx<T>), modes, hipMemcpyDeviceToHost, s),
-                  "hipMemcpy get_uhat");
-        hip_check(hipStreamSynchronize(s), "get_uhat");
+        const std::vector<cx<T>> tmp = download_state(m, s);
         for (std::int64_t i = 0; i < modes; ++i) {
-            host[2 * i] = tmp[i].x;
-            host[2 * i + 1] = tmp[i].y;
+            host[2 * i] = tmp[3 * i + c].x;
+            host[2 * i + 1] = tmp[3 * i + c].y;
         }
     }
 
     void set_uhat(int m, int c, const double* host, hipStream_t s) {
         if (c < 0 || c > 2) throw std::runtime_error("component out of range");
-        std::vector<cx<T>> tmp(static_cast<std::size_t>(modes));
-        for (std::int64_t i = 0; i < modes; ++i) tmp[i] = {static_cast<T>(host[2 * i]), static_cast<T>(host[2 * i + 1])};
-        hip_check(hipMemcpy2DAsync(U_.get() + m * su + c, 3 * sizeof(cx<T>), tmp.data(), sizeof(cx<T>),
-                                   sizeof(cx<T>), modes, hipMemcpyHostToDevice, s),
+        std::vector<cx<T>> tmp = download_state(m, s);
+        for (std::int64_t i = 0; i < modes; ++i)
+            tmp[3 * i + c] = {static_cast<T>(host[2 * i]), static_cast<T>(host[2 * i + 1])};
+        hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
                   "hipMemcpy set_uhat");
         hip_check(hipStreamSynchronize(s), "set_uhat");
+    }
+
+    // all three components of member m at once
+    void set_state(int m, const double* const* u, hipStream_t s) {
+        std::vector<cx<T>> tmp(static_cast<std::size_t>(su));
+        for (std::int64_t i = 0; i < modes; ++i)
+            for (int c = 0; c < 3; ++c)
+                tmp[3 * i + c] = {static_cast<T>(u[c][2 * i]), static_cast<T>(u[c][2 * i + 1])};
+        hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
+                  "hipMemcpy set_state");
+        hip_check(hipStreamSynchronize(s), "set_state");
     }
 
     // impl.hpp:796-817 for member m: TG field at cell centres, 3 forward
     // transforms, project_truncate
     void taylor_green(int m, hipStream_t s) {
         const std::int64_t M = modes;
-        T* P = P_.get() + m * sp;
+        T* P = P_.get();
         cx<T>* W = W_.get() + m * sw;
         k_taylor_green<T><<<2048, 256, 0, s>>>(P, P + npts, P + 2 * npts, n);
         FwdXArgs<T> fx{};
@@ -421,7 +415,7 @@ public:
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "hipEventCreate");
         hip_check(hipEventCreate(&b), "hipEventCreate");
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < kPasses; ++k) {
             // warm-up; the passes before k make its inputs genuine
             for (int j = 0; j <= k; ++j) launch_pass(j, 1, u2.get(), g2.get(), 0, count, s);
             hip_check(hipEventRecord(a, s), "hipEventRecord");
@@ -435,14 +429,23 @@ public:
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         hip_check(hipGetLastError(), "time_passes");
+        // algorithmic bytes (each array read or written once), C = one
+        // complex half-spectrum field
         const double C = static_cast<double>(modes) * 2.0 * sizeof(T) * count;
-        const double R = static_cast<double>(npts) * sizeof(T) * count;
-        const double b6[6] = {9 * C, 12 * C, 6 * C + 6 * R, 6 * R + 12 * C, 24 * C, 24 * C};
-        for (int k = 0; k < 6; ++k) bytes[k] = b6[k];
+        const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 24 * C};
+        for (int k = 0; k < kPasses; ++k) bytes[k] = b5[k];
         hip_check(hipStreamSynchronize(s), "time_passes");
     }
 
 private:
+    std::vector<cx<T>> download_state(int m, hipStream_t s) const {
+        std::vector<cx<T>> tmp(static_cast<std::size_t>(su));
+        hip_check(hipMemcpyAsync(tmp.data(), U_.get() + m * su, su * sizeof(cx<T>), hipMemcpyDeviceToHost, s),
+                  "hipMemcpy get_uhat");
+        hip_check(hipStreamSynchronize(s), "get_uhat");
+        return tmp;
+    }
+
     const PassTable<T>* pt_ = nullptr;
     DeviceArray<cx<T>> U_, G_, W_, tw_, phase_;
     DeviceArray<T> P_, coef_;
@@ -656,12 +659,13 @@ public:
             const std::size_t modes = static_cast<std::size_t>(nj / 2 + 1) * nj * nj;
             std::vector<double> u0(2 * modes), u1(2 * modes), u2(2 * modes);
             random_solenoidal_field(nj, seeds[j], energy, u0.data(), u1.data(), u2.data());
-            set_uhat(j, 0, u0.data());
-            set_uhat(j, 1, u1.data());
-            set_uhat(j, 2, u2.data());
+            const double* u[3] = {u0.data(), u1.data(), u2.data()};
+            const auto w = locate(j);
+            groups_[w.first]->set_state(w.second, u, stream_.get());
         }
         dt_.resize(groups_.size());
         for (std::size_t q = 0; q < groups_.size(); ++q) dt_[q].assign(groups_[q]->count, 0.0);
+        make_branches();
     }
 
     void step(const double* dt) override {
@@ -669,9 +673,22 @@ public:
         for (std::size_t j = 0; j < where_.size(); ++j) dt_[where_[j].first][where_[j].second] = dt[j];
         for (std::size_t q = 0; q < groups_.size(); ++q) groups_[q]->set_dt(dt_[q].data(), s);
         auto ev = timer_.begin(s);
+        // one branch per size: the groups' chains of 15 launches are
+        // independent, so small grids run beside large ones
         auto record = [&] {
-            for (int st = 0; st < 3; ++st)
-                for (auto& g : groups_) g->launch_stage(st, s);
+            if (groups_.size() == 1) {
+                for (int st = 0; st < 3; ++st) groups_[0]->launch_stage(st, s);
+                return;
+            }
+            hip_check(hipEventRecord(fork_, s), "hipEventRecord");
+            for (std::size_t q = 0; q < groups_.size(); ++q) {
+                const hipStream_t b = branch_[q]->get();
+                hip_check(hipStreamWaitEvent(b, fork_, 0), "hipStreamWaitEvent");
+                for (int st = 0; st < 3; ++st) groups_[q]->launch_stage(st, b);
+                hip_check(hipEventRecord(join_[q], b), "hipEventRecord");
+            }
+            for (std::size_t q = 0; q < groups_.size(); ++q)
+                hip_check(hipStreamWaitEvent(s, join_[q], 0), "hipStreamWaitEvent");
         };
         if (use_graph_) {
             graph_.launch(s, record);
@@ -704,16 +721,16 @@ public:
         if (steps) *steps = timer_.steps;
         return timer_.seconds();
     }
-    // Per pass, summed over the groups (the ensemble's six launches per stage
-    // per group).
+    // Per pass, summed over the groups (one launch per pass, stage and group).
     void time_passes(int reps, double* ms, double* bytes) override {
-        for (int k = 0; k < 6; ++k) ms[k] = bytes[k] = 0.0;
+        constexpr int K = Group<T>::kPasses;
+        for (int k = 0; k < K; ++k) ms[k] = bytes[k] = 0.0;
         for (auto& g : groups_) {
-            double m6[6], b6[6];
-            g->time_passes(reps, m6, b6, stream_.get());
-            for (int k = 0; k < 6; ++k) {
-                ms[k] += m6[k];
-                bytes[k] += b6[k];
+            double mk[K], bk[K];
+            g->time_passes(reps, mk, bk, stream_.get());
+            for (int k = 0; k < K; ++k) {
+                ms[k] += mk[k];
+                bytes[k] += bk[k];
             }
         }
     }
@@ -722,6 +739,14 @@ private:
     std::pair<int, int> locate(int member) const {
         if (member < 0 || member >= static_cast<int>(where_.size())) throw std::runtime_error("member out of range");
         return where_[member];
+    }
+
+    void make_branches() {
+        if (groups_.size() < 2) return;
+        hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+        join_.assign(groups_.size(), nullptr);
+        for (auto& e : join_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        for (std::size_t q = 0; q < groups_.size(); ++q) branch_.push_back(std::make_unique<Stream>());
     }
 
     Stream stream_;
@@ -733,6 +758,19 @@ private:
     bool use_graph_ = true;
     StepGraph graph_;
     StepTimer timer_;
+    // fork/join of the per-size branches (default-priority streams)
+    std::vector<std::unique_ptr<Stream>> branch_;
+    hipEvent_t fork_ = nullptr;
+    std::vector<hipEvent_t> join_;
+
+public:
+    ~Ensemble() override {
+        (void)hipStreamSynchronize(stream_.get());
+        for (auto& b : branch_) (void)hipStreamSynchronize(b->get());
+        if (fork_) (void)hipEventDestroy(fork_);
+        for (auto e : join_)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 }  // namespace sph

@@ -84,3 +84,11 @@ def test_null_handles_are_rejected():
 def test_no_device_fails_loudly():
     with pytest.raises(RuntimeError, match="no HIP device"):
         sph.TaylorGreenSolver(16, 1600.0)
+
+
+def test_pass_count_matches_header():
+    """The timing calls write one double per pass; the binding's arrays must
+    be that long."""
+    text = open(HEADER).read()
+    m = re.search(r"written to ms\[(\d+)\]", text)
+    assert m and int(m.group(1)) == len(sph._PASS_NAMES)
