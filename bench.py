@@ -171,6 +171,8 @@ def main():
     s32, sec32 = time_steps(sph, n, "fp32", K, W, args.dt)
     passes = s32.time_passes(args.reps)
     log("pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
+    tuning = s32.tuning()
+    log("autotuned tile lines (ms per width):", tuning)
 
     fp64 = None
     eps_err = None
@@ -195,7 +197,8 @@ def main():
                 "kernel": f"{PASS_NAMES[dom]} ({KERNEL_NAMES[dom]})", "launch_ms": round(ms, 4),
                 "algorithmic_bytes": by,
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
-                "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()}}
+                "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()},
+                "tile_lines": {k: v[0] for k, v in tuning.items()}}
     s32.close()
 
     copy_gbs = sph.copy_bandwidth(2 << 30, 10)

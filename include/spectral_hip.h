@@ -159,6 +159,14 @@ int sph_solver_synchronize(sph_solver_t solver);
  * each pass in bytes[5]. The state is unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
+/* The choices of the solver's tile autotuner (the reference plans its FFTs
+ * with FFTW_ESTIMATE / cuFFT, impl.hpp:257-412; here the y and x passes come
+ * in two tile widths timed at creation on the solver's own buffers). For the
+ * tuned passes inverse-y, x, forward-y: lines[3] = lines per tile kept, and
+ * ms[6] = mean launch time of each width, {pass0 width0, pass0 width1, ...}
+ * (zeros if tuning was switched off with SPH_AUTOTUNE=0). */
+int sph_solver_tuning(sph_solver_t solver, int* lines, double* ms);
+
 /* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,
  * timed with HIP events over `reps` launches; *gbs = 2*bytes/time (read +
  * write) in GB/s. The measured ceiling against which the pass kernels are

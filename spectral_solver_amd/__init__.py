@@ -86,6 +86,7 @@ SYMBOLS = [
     ("sph_solver_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sph_solver_synchronize", ctypes.c_int, [_VP]),
     ("sph_solver_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
+    ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _DP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
     ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
@@ -357,6 +358,14 @@ class TaylorGreenSolver:
         by = np.zeros(len(_PASS_NAMES))
         _check(load_library().sph_solver_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
+
+    def tuning(self) -> dict:
+        """Autotuner choice per tuned pass: (lines per tile, ms of each width)."""
+        lines = np.zeros(3, np.int32)
+        ms = np.zeros(6)
+        _check(load_library().sph_solver_tuning(self._h, lines.ctypes.data_as(_IP), _dptr(ms)))
+        names = ("inverse_y", "x_products", "forward_y")
+        return {k: (int(lines[p]), [float(ms[2 * p]), float(ms[2 * p + 1])]) for p, k in enumerate(names)}
 
     def u_hat(self, c: int) -> np.ndarray:
         """Component c as complex [kx in 0..n/2][kz][ky]."""

@@ -110,10 +110,17 @@ constexpr RadixPlan z_radix_plan(int n) {
 constexpr int round_up(int v, int m) { return (v + m - 1) / m * m; }
 constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Default lines per tile: one 128-byte row segment of complex values per
+// element. The pair passes also run narrower tiles (see PassTable).
 template <typename T, int N>
+constexpr int default_lines() {
+    return sizeof(T) == 4 ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8);
+}
+
+template <typename T, int N, int NL_ = default_lines<T, N>()>
 struct Tile {
-    // lines per tile: one 128-byte row segment of complex values per element
-    static constexpr int NL = sizeof(T) == 4 ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8);
+    static constexpr int NL = NL_;
+    static_assert(NL >= 2 && NL % 2 == 0, "tiles hold an even number of lines");
     static constexpr int STRIDE = NL + 1;
     static constexpr int ELEMS = NL * N;
     static constexpr int NT = clampi(round_up(ELEMS / 8, 64), 64, 1024);

@@ -414,6 +414,15 @@ int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* by
     });
 }
 
+int sph_solver_tuning(sph_solver_t solver, int* lines, double* ms) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(lines);
+        require_pointer(ms);
+        solver->impl->tuning(lines, ms);
+    });
+}
+
 int sph_measure_copy_bandwidth(int64_t bytes, int reps, double* gbs) {
     return guarded([&] {
         require_pointer(gbs);

@@ -194,10 +194,10 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
 // contiguous y lines of pair fields, in place: a tile is NL/2 pair lines,
 // i.e. NL*N consecutive complex values, member f of pair line g is line 2g+f
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool INV>
-__global__ void __launch_bounds__((Tile<T, N>::NT))
+template <typename T, int N, bool INV, int LINES = default_lines<T, N>()>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     k_plines(cx<T>* data, long long npairlines, long long member_stride, const cx<T>* tw) {
-    using TL = Tile<T, N>;
+    using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
     __shared__ cx<T> buf[TL::LDS];
@@ -231,10 +231,10 @@ __global__ void __launch_bounds__((Tile<T, N>::NT))
 
 // Spectral x lines of pair field AB -> buf as Z = A + iB over all N modes
 // (Hermitian extension of both members; Re rule on kx = 0 and N/2).
-template <typename T, int N>
+template <typename T, int N, int LINES>
 __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, int tid, int y0, int z, int ny,
                                                   int nz) {
-    using TL = Tile<T, N>;
+    using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
     constexpr int LOADS = NL * (N / 2 + 1);
     constexpr int LPT = (LOADS + NT - 1) / NT;
@@ -263,11 +263,11 @@ __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, i
 // The six products (u_a u_b, s_a s_b), ab in (00,01,02,11,12,22), of the
 // physical values held in registers: forward x transform of each pair and
 // separation into the pair field out[p] (P_hat, Q_hat) at the tile's modes.
-template <typename T, int N, int E>
+template <typename T, int N, int LINES, int E>
 __device__ __forceinline__ void products_forward_x(const T (&u)[3][E], const T (&s)[3][E], cx<T>* buf,
                                                    const cx<T>* twl, cx<T>* const (&out)[6], long long zw, int tid,
                                                    int y0, int z, int ny, int nz) {
-    using TL = Tile<T, N>;
+    using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
     static_assert(E == TL::E, "register slots must match the tile");
     constexpr int ITEMS = NL * (N / 2 + 1);
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
     load_twiddles<T, N, NT>(twl, a.tw, tid);
 #pragma unroll 1
     for (int p = 0; p < a.npairs; ++p) {
-        load_pair_x_lines<T, N>(buf, pick(a.ab, p) + zw, tid, y0, z, a.ny, a.nz);
+        load_pair_x_lines<T, N, NL>(buf, pick(a.ab, p) + zw, tid, y0, z, a.ny, a.nz);
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
         T* OA = pick(a.oa, p) + zp;
@@ -362,9 +362,9 @@ struct PairXArgs {
     long long ms_w;
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N>::NT)) k_px_products(PairXArgs<T> a) {
-    using TL = Tile<T, N>;
+template <typename T, int N, int LINES = default_lines<T, N>()>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXArgs<T> a) {
+    using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
     __shared__ cx<T> buf[TL::LDS];
     __shared__ cx<T> twl[N];
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_px_products(PairXArgs<T> a
     T u[3][E], s[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        load_pair_x_lines<T, N>(buf, a.ab[c] + zw, tid, y0, z, a.ny, a.nz);
+        load_pair_x_lines<T, N, NL>(buf, a.ab[c] + zw, tid, y0, z, a.ny, a.nz);
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
 #pragma unroll
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_px_products(PairXArgs<T> a
         }
         __syncthreads();
     }
-    products_forward_x<T, N, E>(u, s, buf, twl, a.out, zw, tid, y0, z, a.ny, a.nz);
+    products_forward_x<T, N, NL, E>(u, s, buf, twl, a.out, zw, tid, y0, z, a.ny, a.nz);
 }
 
 // ---------------------------------------------------------------------------

@@ -7,11 +7,17 @@
 
 namespace sph {
 
+// Tile variants of the pair y and x passes: variant v runs tiles of
+// tile_lines[v] lines (the default width, then half of it). The solver times
+// both on its own buffers and keeps the faster (Group::autotune).
+constexpr int kTileVariants = 2;
+
 template <typename T>
 struct PassTable {
     int n;   // transform length
     int nl;  // lines per tile
     int nt;  // threads per work-group
+    int tile_lines[kTileVariants];
     void (*inv_z)(const InvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
     void (*lines)(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t);
     void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
@@ -20,9 +26,9 @@ struct PassTable {
     // solver stage on pair-interleaved buffers
     void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, int members, hipStream_t);
     void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
-                   const cx<T>* tw, hipStream_t);
+                   const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
-    void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, hipStream_t);
+    void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, hipStream_t);
     void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, int members, hipStream_t);
 };
 
@@ -84,28 +90,46 @@ struct Launch {
         else
             k_pinv_z<T, N, PZ_VEL><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static void plines(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
-                       const cx<T>* tw, hipStream_t s) {
-        const dim3 grid(static_cast<unsigned>((npairlines + TL::NL / 2 - 1) / (TL::NL / 2)), members);
+    static constexpr int L0 = TL::NL;
+    static constexpr int L1 = L0 / 2 >= 2 ? L0 / 2 : L0;
+    template <int L>
+    static void plines_l(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
+                         const cx<T>* tw, hipStream_t s) {
+        using TV = Tile<T, N, L>;
+        const dim3 grid(static_cast<unsigned>((npairlines + L / 2 - 1) / (L / 2)), members);
         if (inverse)
-            k_plines<T, N, true><<<grid, TL::NT, 0, s>>>(data, npairlines, member_stride, tw);
+            k_plines<T, N, true, L><<<grid, TV::NT, 0, s>>>(data, npairlines, member_stride, tw);
         else
-            k_plines<T, N, false><<<grid, TL::NT, 0, s>>>(data, npairlines, member_stride, tw);
+            k_plines<T, N, false, L><<<grid, TV::NT, 0, s>>>(data, npairlines, member_stride, tw);
+    }
+    static void plines(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
+                       const cx<T>* tw, int variant, hipStream_t s) {
+        if (variant == 1)
+            plines_l<L1>(data, npairlines, member_stride, members, inverse, tw, s);
+        else
+            plines_l<L0>(data, npairlines, member_stride, members, inverse, tw, s);
     }
     static void pinv_x(const PairInvXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
         k_pinv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
-    static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
-        const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
-        k_px_products<T, N><<<grid, TL::NT, 0, s>>>(a);
+    template <int L>
+    static void px_products_l(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+        const dim3 grid((ny + L - 1) / L, nz, members);
+        k_px_products<T, N, L><<<grid, Tile<T, N, L>::NT, 0, s>>>(a);
+    }
+    static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
+        if (variant == 1)
+            px_products_l<L1>(a, ny, nz, members, s);
+        else
+            px_products_l<L0>(a, ny, nz, members, s);
     }
     static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, int members, hipStream_t s) {
         const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
         k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static constexpr PassTable<T> table{N,     TL::NL,  TL::NT,  &inv_z,  &lines,  &inv_x,    &fwd_x,
-                                        &fwd_z, &pinv_z, &plines, &pinv_x, &px_products, &pfwd_z_rk};
+    static constexpr PassTable<T> table{N,       TL::NL,  TL::NT,  {L0, L1},     &inv_z,    &lines, &inv_x,
+                                        &fwd_x,  &fwd_z,  &pinv_z, &plines, &pinv_x, &px_products, &pfwd_z_rk};
 };
 
 }  // namespace sph

@@ -12,10 +12,14 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -167,11 +171,24 @@ public:
         upload(phase_, phase_table<T>(n));
         hip_check(hipMemset(G_.get(), 0, count * su * sizeof(cx<T>)), "hipMemset");
         hip_check(hipMemset(U_.get(), 0, count * su * sizeof(cx<T>)), "hipMemset");
+        hip_check(hipMemset(W_.get(), 0, count * sw * sizeof(cx<T>)), "hipMemset");
         // the solver streams are non-blocking: finish the clears first
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         nu_.resize(count);
         for (int m = 0; m < count; ++m) nu_[m] = static_cast<T>(nu[m]);
         last_dt_.assign(count, std::nan(""));
+        autotune();
+    }
+
+    // Tuned passes: inverse y, x pass, forward y.
+    static constexpr int kTuned = 3;
+    // lines per tile of each tuned pass, and the mean launch time of each
+    // tile variant (ms[pass * kTileVariants + variant]; 0 when not timed)
+    void tuning(int* lines, double* ms) const {
+        for (int p = 0; p < kTuned; ++p) {
+            lines[p] = pt_->tile_lines[var_[p]];
+            for (int v = 0; v < kTileVariants; ++v) ms[p * kTileVariants + v] = tune_ms_[p][v];
+        }
     }
 
     static constexpr int kPasses = 5;  // kernel launches per RK stage
@@ -220,7 +237,7 @@ public:
                 break;
             }
             case 1:
-                pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), s);
+                pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
             case 2: {
                 PairXArgs<T> a{};
@@ -231,11 +248,11 @@ public:
                 a.scale = T(1) / static_cast<T>(npts);
                 a.tw = tw_.get();
                 a.ms_w = sw;
-                pt_->px_products(a, n, n, members, s);
+                pt_->px_products(a, n, n, members, var_[1], s);
                 break;
             }
             case 3:
-                pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), s);
+                pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
                 break;
             default: {
                 PairFwdZArgs<T> a{};
@@ -280,7 +297,7 @@ public:
         x.tw = tw_.get();
         if (mode == PZ_DIAG) {
             pt_->pinv_z(a, PZ_DIAG, n, H, 1, s);
-            pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), s);
+            pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             for (int c = 0; c < 3; ++c) {
                 x.ab[c] = W + c * M2;
                 x.oa[c] = P + c * npts;
@@ -289,7 +306,7 @@ public:
             x.npairs = 3;
         } else {
             pt_->pinv_z(a, PZ_VEL, n, H, 1, s);
-            pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), s);
+            pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             x.ab[0] = W;
             x.oa[0] = P;
             x.ob[0] = P + npts;
@@ -438,6 +455,72 @@ public:
     }
 
 private:
+    // Tile-width autotuner: each variant of the tuned passes is launched on
+    // this group's (zeroed) work buffer, one warm-up and three timed launches
+    // between HIP events, and the faster is kept. Results are cached per
+    // (precision, n, members) for the process; SPH_AUTOTUNE=0 keeps the
+    // default widths, SPH_TILE_VARIANT=v forces variant v everywhere.
+    void autotune() {
+        using Key = std::tuple<int, int, int>;
+        struct Choice {
+            int var[kTuned];
+            double ms[kTuned][kTileVariants];
+        };
+        const char* force = std::getenv("SPH_TILE_VARIANT");
+        if (force && force[0] >= '0' && force[0] < '0' + kTileVariants && force[1] == '\0') {
+            for (int p = 0; p < kTuned; ++p) var_[p] = force[0] - '0';
+            return;
+        }
+        static std::mutex mu;
+        static std::map<Key, Choice> cache;
+        const Key key{static_cast<int>(sizeof(T)), n, count};
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            const auto it = cache.find(key);
+            if (it != cache.end()) {
+                for (int p = 0; p < kTuned; ++p) {
+                    var_[p] = it->second.var[p];
+                    for (int v = 0; v < kTileVariants; ++v) tune_ms_[p][v] = it->second.ms[p][v];
+                }
+                return;
+            }
+        }
+        const char* env = std::getenv("SPH_AUTOTUNE");
+        if (env && env[0] == '0') return;
+        Stream stream;
+        const hipStream_t s = stream.get();
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        const int pass_of[kTuned] = {1, 2, 3};
+        for (int p = 0; p < kTuned; ++p) {
+            for (int v = 0; v < kTileVariants; ++v) {
+                var_[p] = v;
+                launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
+                hip_check(hipEventRecord(a, s), "hipEventRecord");
+                for (int r = 0; r < 3; ++r) launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
+                hip_check(hipEventRecord(b, s), "hipEventRecord");
+                hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+                float e = 0.f;
+                hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
+                tune_ms_[p][v] = e / 3.0;
+            }
+            var_[p] = tune_ms_[p][1] < tune_ms_[p][0] ? 1 : 0;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        // the timing launches left transforms of zeros in W: clear it again
+        hip_check(hipMemsetAsync(W_.get(), 0, count * sw * sizeof(cx<T>), s), "hipMemset");
+        hip_check(hipStreamSynchronize(s), "autotune");
+        Choice c{};
+        for (int p = 0; p < kTuned; ++p) {
+            c.var[p] = var_[p];
+            for (int v = 0; v < kTileVariants; ++v) c.ms[p][v] = tune_ms_[p][v];
+        }
+        std::lock_guard<std::mutex> lock(mu);
+        cache.emplace(key, c);
+    }
+
     std::vector<cx<T>> download_state(int m, hipStream_t s) const {
         std::vector<cx<T>> tmp(static_cast<std::size_t>(su));
         hip_check(hipMemcpyAsync(tmp.data(), U_.get() + m * su, su * sizeof(cx<T>), hipMemcpyDeviceToHost, s),
@@ -452,6 +535,8 @@ private:
     DeviceArray<double> partial_, result_;
     std::vector<T> nu_;
     std::vector<double> last_dt_;
+    int var_[kTuned] = {0, 0, 0};  // tile variant of each tuned pass
+    double tune_ms_[kTuned][kTileVariants] = {};
 };
 
 // Per-step GPU time from event pairs, resolved lazily.
@@ -538,6 +623,7 @@ public:
     virtual void set_graph(bool on) = 0;
     virtual void synchronize() = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
+    virtual void tuning(int* lines, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
 };
@@ -601,6 +687,7 @@ public:
     void set_graph(bool on) override { use_graph_ = on; }
     void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }
     void time_passes(int reps, double* ms, double* bytes) override { g_.time_passes(reps, ms, bytes, stream_.get()); }
+    void tuning(int* lines, double* ms) const override { g_.tuning(lines, ms); }
 
 private:
     Stream stream_;
