@@ -172,7 +172,7 @@ def main():
     passes = s32.time_passes(args.reps)
     log("pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
     tuning = s32.tuning()
-    log("autotuned tile lines (ms per width):", tuning)
+    log("autotuner (variant, lines per tile, ms per variant):", tuning)
 
     fp64 = None
     eps_err = None
@@ -198,7 +198,7 @@ def main():
                 "algorithmic_bytes": by,
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()},
-                "tile_lines": {k: v[0] for k, v in tuning.items()}}
+                "autotuned": {k: {"variant": v[0], "lines": v[1]} for k, v in tuning.items()}}
     s32.close()
 
     copy_gbs = sph.copy_bandwidth(2 << 30, 10)

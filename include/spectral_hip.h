@@ -159,13 +159,17 @@ int sph_solver_synchronize(sph_solver_t solver);
  * each pass in bytes[5]. The state is unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
-/* The choices of the solver's tile autotuner (the reference plans its FFTs
- * with FFTW_ESTIMATE / cuFFT, impl.hpp:257-412; here the y and x passes come
- * in two tile widths timed at creation on the solver's own buffers). For the
- * tuned passes inverse-y, x, forward-y: lines[3] = lines per tile kept, and
- * ms[6] = mean launch time of each width, {pass0 width0, pass0 width1, ...}
- * (zeros if tuning was switched off with SPH_AUTOTUNE=0). */
-int sph_solver_tuning(sph_solver_t solver, int* lines, double* ms);
+/* The choices of the solver's autotuner (the reference plans its FFTs with
+ * FFTW_ESTIMATE / cuFFT, impl.hpp:257-412; here the y and x passes come in
+ * several tile widths and kernel forms, timed at creation on the solver's own
+ * buffers). For the tuned passes inverse-y, x, forward-y: variant[3] = the
+ * variant kept, lines[3] = its lines per tile, ms[9] = mean launch time of
+ * each variant, {pass0 v0, pass0 v1, pass0 v2, pass1 v0, ...} (0 where a pass
+ * has fewer variants, or tuning was switched off with SPH_AUTOTUNE=0).
+ * Variants: y passes 0 = default width, 1 = half width; x pass
+ * 0 = register-direct default width, 1 = register-direct half width,
+ * 2 = LDS-staged default width. */
+int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms);
 
 /* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,
  * timed with HIP events over `reps` launches; *gbs = 2*bytes/time (read +

@@ -86,7 +86,7 @@ SYMBOLS = [
     ("sph_solver_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sph_solver_synchronize", ctypes.c_int, [_VP]),
     ("sph_solver_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
-    ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _DP]),
+    ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _IP, _DP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
     ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
@@ -360,12 +360,15 @@ class TaylorGreenSolver:
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 
     def tuning(self) -> dict:
-        """Autotuner choice per tuned pass: (lines per tile, ms of each width)."""
+        """Autotuner choice per tuned pass: (variant, lines per tile, ms of
+        each variant)."""
+        var = np.zeros(3, np.int32)
         lines = np.zeros(3, np.int32)
-        ms = np.zeros(6)
-        _check(load_library().sph_solver_tuning(self._h, lines.ctypes.data_as(_IP), _dptr(ms)))
+        ms = np.zeros(9)
+        _check(load_library().sph_solver_tuning(self._h, var.ctypes.data_as(_IP), lines.ctypes.data_as(_IP),
+                                                _dptr(ms)))
         names = ("inverse_y", "x_products", "forward_y")
-        return {k: (int(lines[p]), [float(ms[2 * p]), float(ms[2 * p + 1])]) for p, k in enumerate(names)}
+        return {k: (int(var[p]), int(lines[p]), [float(x) for x in ms[3 * p:3 * p + 3]]) for p, k in enumerate(names)}
 
     def u_hat(self, c: int) -> np.ndarray:
         """Component c as complex [kx in 0..n/2][kz][ky]."""

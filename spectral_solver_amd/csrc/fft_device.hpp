@@ -267,7 +267,7 @@ __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) 
 // stage leaves element j + r*N/RL of line l in registers for the caller's
 // post-operation and coalesced store.
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, int NL_ = 0>  // NL_ = 0: the default width
 struct ZTile {
     static constexpr RadixPlan P = z_radix_plan(N);
     static constexpr int lcm_radix() {
@@ -289,7 +289,7 @@ struct ZTile {
         while (nl * N / lcm_radix() < 64) nl *= 2;
         return nl;
     }
-    static constexpr int NL = pick_nl();
+    static constexpr int NL = NL_ ? NL_ : pick_nl();
     static constexpr int E = lcm_radix();  // elements per thread and field
     static constexpr int NT = NL * N / E;
     static constexpr int S = NL + 1;           // LDS row stride
@@ -357,11 +357,12 @@ __device__ __forceinline__ void zstages_mid(cx<T>* lds, const cx<T>* twl, int ti
     }
 }
 
-// v: first-stage inputs (consumed), w: last-stage outputs.
-template <typename T, int N, int NF, bool INV, int A0, int B0, int A1, int B1>
+// v: first-stage inputs (consumed), w: last-stage outputs. LINES: tile width
+// (0 = the default ZTile width).
+template <typename T, int N, int NF, bool INV, int LINES = 0, int A0, int B0, int A1, int B1>
 __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
                                      int tid) {
-    using Z = ZTile<T, N>;
+    using Z = ZTile<T, N, LINES>;
     static_assert(A0 == Z::T0 && B0 == Z::R0 && A1 == Z::TLAST && B1 == Z::RL, "slot arrays must match the tile");
     constexpr int NL = Z::NL, NT = Z::NT, S = Z::S, R0 = Z::R0, RL = Z::RL;
     constexpr RadixPlan P = Z::P;

@@ -414,12 +414,13 @@ int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* by
     });
 }
 
-int sph_solver_tuning(sph_solver_t solver, int* lines, double* ms) {
+int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms) {
     return guarded([&] {
         require_handle(solver);
+        require_pointer(variant);
         require_pointer(lines);
         require_pointer(ms);
-        solver->impl->tuning(lines, ms);
+        solver->impl->tuning(variant, lines, ms);
     });
 }
 

@@ -95,18 +95,19 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
     __shared__ cx<T> lds[Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
     const int l = tid % NL, f = l & 1;
-    const int ky = blockIdx.x * (NL / 2) + (l >> 1);
-    const int kx = blockIdx.y;
+    const int ky = tile.x * (NL / 2) + (l >> 1);
+    const int kx = tile.y;
     const bool ok = ky < a.ny;
     const uint32_t kyc = ok ? ky : a.ny - 1;
     constexpr uint32_t CB = sizeof(cx<T>);
     const uint32_t ny = a.ny;
     // member offsets kept in locals: writing into the argument struct would
     // copy it to private memory
-    const long long zw = blockIdx.z * a.ms_w;
+    const long long zw = tile.z * a.ms_w;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
-    const Buf<T> U(a.u3 + blockIdx.z * a.ms_u + plane * 3, N * ny * 3 * CB);
+    const Buf<T> U(a.u3 + tile.z * a.ms_u + plane * 3, N * ny * 3 * CB);
     const uint32_t u_lane = ((tid / NL) * ny + kyc) * 3 * CB, u_row = ny * 3 * CB;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
@@ -362,6 +363,105 @@ struct PairXArgs {
     long long ms_w;
 };
 
+// Register-direct form: the x lines are transformed with the z-pass machinery
+// (zfft), lanes over y. The inverse first stage loads its rows straight from
+// HBM (rows k > N/2 from row N-k, conjugated: the Hermitian extension), the
+// inverse last stage leaves u and s in registers at the slots the forward
+// first stage reads (both hold elements j + m*N/E of the lane's line), and
+// only the forward output goes through LDS once more, for the pair separation
+// of Z_k and Z_{N-k}. Same formulas as the LDS-staged form below (results
+// agree to rounding; the compiler may contract multiply-adds differently).
+template <typename T, int N, int LINES = 0>
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT)) k_px_products_rd(PairXArgs<T> a) {
+    using Z = ZTile<T, N, LINES>;
+    constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
+    constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const int l = tid % NL;
+    const int y0 = blockIdx.x * NL;
+    const int z = blockIdx.y;
+    const bool ok = y0 + l < a.ny;
+    const long long zw = blockIdx.z * a.ms_w;
+    const size_t row = static_cast<size_t>(a.nz) * a.ny;  // cx2 elements between k rows
+    const size_t lane = static_cast<size_t>(z) * a.ny + (ok ? y0 + l : 0);
+    load_twiddles<T, N, NT>(twl, a.tw, tid);
+    // u, s at slot m of element j + m*N/E (j = tid / NL)
+    T u[3][E], s[3][E];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw);
+        cx<T> v[1][T0][R0], w[1][TLS][RL];
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int e = Z::e_first(tid, i, r);
+                const int k = e <= N / 2 ? e : N - e;
+                cx<T> va = czero<T>(), vb = czero<T>();
+                if (ok) {
+                    const cx2<T> q = AB[k * row + lane];
+                    va = q.a;
+                    vb = q.b;
+                }
+                if (k == 0 || k == N / 2) {
+                    va.y = T(0);
+                    vb.y = T(0);
+                }
+                v[0][i][r] = e <= N / 2 ? cx<T>{va.x - vb.y, va.y + vb.x} : cx<T>{va.x + vb.y, vb.x - va.y};
+            }
+        zfft<T, N, 1, true, LINES>(v, w, lds, twl, tid);
+#pragma unroll
+        for (int i = 0; i < TLS; ++i)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                u[c][i + r * TLS] = w[0][i][r].x * a.scale;
+                s[c][i + r * TLS] = w[0][i][r].y * a.scale;
+            }
+    }
+    constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
+    constexpr int ITEMS = NL * (N / 2 + 1);
+    constexpr int IPT = (ITEMS + NT - 1) / NT;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        cx<T> v[1][T0][R0], w[1][TLS][RL];
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int m = i + r * T0;
+                v[0][i][r] = {u[PA[p]][m] * u[PB[p]][m], s[PA[p]][m] * s[PB[p]][m]};
+            }
+        zfft<T, N, 1, false, LINES>(v, w, lds, twl, tid);
+        __syncthreads();  // the last stage's reads of the tile are done
+#pragma unroll
+        for (int i = 0; i < TLS; ++i)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) lds[Z::e_last(tid, i, r) * S + l] = w[0][i][r];
+        __syncthreads();
+        cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw);
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const int idx = tid + i * NT;
+            if (ITEMS % NT == 0 || idx < ITEMS) {
+                const int l2 = idx % NL, k = idx / NL;
+                if (y0 + l2 < a.ny) {
+                    const cx<T> zk = lds[k * S + l2];
+                    const cx<T> zm = lds[((N - k) % N) * S + l2];
+                    cx2<T> q;
+                    q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+                    q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+                    o[k * row + static_cast<size_t>(z) * a.ny + y0 + l2] = q;
+                }
+            }
+        }
+        // the next product's zfft synchronises before it writes the tile
+    }
+}
+
+// LDS-staged form: each transform stages its whole tile through LDS.
 template <typename T, int N, int LINES = default_lines<T, N>()>
 __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXArgs<T> a) {
     using TL = Tile<T, N, LINES>;
@@ -406,17 +506,18 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
     __shared__ cx<T> lds[Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
     const int l = tid % NL, f = l & 1;
-    const int ky = blockIdx.x * (NL / 2) + (l >> 1);
-    const int kx = blockIdx.y;
+    const int ky = tile.x * (NL / 2) + (l >> 1);
+    const int kx = tile.y;
     const bool ok = ky < a.ny;
     const uint32_t kyc = ok ? ky : a.ny - 1;
     constexpr uint32_t CB = sizeof(cx<T>);
     const uint32_t ny = a.ny;
-    const long long zw = blockIdx.z * a.ms_w;
-    cx<T>* const u3 = a.u3 + blockIdx.z * a.ms_u;
-    cx<T>* const g3 = a.g3 + blockIdx.z * a.ms_u;
-    const T* const coef = a.coef + blockIdx.z * kCoefStride;
+    const long long zw = tile.z * a.ms_w;
+    cx<T>* const u3 = a.u3 + tile.z * a.ms_u;
+    cx<T>* const g3 = a.g3 + tile.z * a.ms_u;
+    const T* const coef = a.coef + tile.z * kCoefStride;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;

@@ -7,17 +7,21 @@
 
 namespace sph {
 
-// Tile variants of the pair y and x passes: variant v runs tiles of
-// tile_lines[v] lines (the default width, then half of it). The solver times
-// both on its own buffers and keeps the faster (Group::autotune).
-constexpr int kTileVariants = 2;
+// Variants of the tuned passes (inverse y, x, forward y) that the solver
+// times on its own buffers, keeping the fastest (Group::autotune):
+//   y passes: LDS-staged tiles of the default width, then half of it;
+//   x pass:   register-direct tiles of the default z-tile width, then half of
+//             it, then the LDS-staged form at the default width.
+constexpr int kTunedPasses = 3;
+constexpr int kMaxVariants = 3;
 
 template <typename T>
 struct PassTable {
     int n;   // transform length
     int nl;  // lines per tile
     int nt;  // threads per work-group
-    int tile_lines[kTileVariants];
+    int variants[kTunedPasses];                // variants of each tuned pass
+    int tile_lines[kTunedPasses][kMaxVariants];  // lines per tile of each variant
     void (*inv_z)(const InvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
     void (*lines)(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t);
     void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
@@ -113,23 +117,45 @@ struct Launch {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz, members);
         k_pinv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
+    // register-direct x tiles: default z-tile width and half of it when that
+    // still makes a whole number of wavefronts
+    static constexpr int X0 = ZT::NL;
+    static constexpr int X1 = X0 / 2 >= 2 && (X0 / 2) * N / ZT::E >= 64 ? X0 / 2 : X0;
     template <int L>
-    static void px_products_l(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+    static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
+        using ZV = ZTile<T, N, L>;
         const dim3 grid((ny + L - 1) / L, nz, members);
-        k_px_products<T, N, L><<<grid, Tile<T, N, L>::NT, 0, s>>>(a);
+        k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
     }
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
-        if (variant == 1)
-            px_products_l<L1>(a, ny, nz, members, s);
-        else
-            px_products_l<L0>(a, ny, nz, members, s);
+        if (variant == 2) {
+            const dim3 grid((ny + L0 - 1) / L0, nz, members);
+            k_px_products<T, N, L0><<<grid, TL::NT, 0, s>>>(a);
+        } else if (variant == 1) {
+            px_products_rd<X1>(a, ny, nz, members, s);
+        } else {
+            px_products_rd<X0>(a, ny, nz, members, s);
+        }
     }
     static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, int members, hipStream_t s) {
         const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
         k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
     }
-    static constexpr PassTable<T> table{N,       TL::NL,  TL::NT,  {L0, L1},     &inv_z,    &lines, &inv_x,
-                                        &fwd_x,  &fwd_z,  &pinv_z, &plines, &pinv_x, &px_products, &pfwd_z_rk};
+    static constexpr PassTable<T> table{N,
+                                        TL::NL,
+                                        TL::NT,
+                                        {2, 3, 2},
+                                        {{L0, L1, 0}, {X0, X1, L0}, {L0, L1, 0}},
+                                        &inv_z,
+                                        &lines,
+                                        &inv_x,
+                                        &fwd_x,
+                                        &fwd_z,
+                                        &pinv_z,
+                                        &plines,
+                                        &pinv_x,
+                                        &px_products,
+                                        &pfwd_z_rk};
 };
 
 }  // namespace sph

@@ -75,6 +75,24 @@ struct Buf {
     }
 };
 
+// Work-groups are dealt to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2. Renumber them so that consecutive logical tiles (x fastest)
+// run on the same XCD at about the same time: tiles whose rows share a 128-byte
+// line (the 24/48-byte-per-mode state rows of the z passes) then meet in one
+// L2 instead of fetching the line twice. Identity on the last total % 8 ids.
+struct TileId {
+    unsigned x, y, z;
+};
+__device__ __forceinline__ TileId xcd_tile_id() {
+    constexpr unsigned kXcds = 8;
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned total = gx * gy * gridDim.z;
+    const unsigned p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned full = total / kXcds * kXcds;
+    const unsigned l = p < full ? (p % kXcds) * (full / kXcds) + p / kXcds : p;
+    return {l % gx, (l / gx) % gy, l / (gx * gy)};
+}
+
 // Pointer q of an argument array, selected with a uniform index: a chain of
 // scalar selects, so the array is never copied to private memory (dynamic
 // indexing of a by-value kernel argument would be).

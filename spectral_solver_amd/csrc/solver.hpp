@@ -181,13 +181,15 @@ public:
     }
 
     // Tuned passes: inverse y, x pass, forward y.
-    static constexpr int kTuned = 3;
-    // lines per tile of each tuned pass, and the mean launch time of each
-    // tile variant (ms[pass * kTileVariants + variant]; 0 when not timed)
-    void tuning(int* lines, double* ms) const {
+    static constexpr int kTuned = kTunedPasses;
+    // per tuned pass: the variant kept, its lines per tile, and the mean
+    // launch time of every variant (ms[pass * kMaxVariants + variant]; 0 when
+    // not timed or not defined)
+    void tuning(int* variant, int* lines, double* ms) const {
         for (int p = 0; p < kTuned; ++p) {
-            lines[p] = pt_->tile_lines[var_[p]];
-            for (int v = 0; v < kTileVariants; ++v) ms[p * kTileVariants + v] = tune_ms_[p][v];
+            variant[p] = var_[p];
+            lines[p] = pt_->tile_lines[p][var_[p]];
+            for (int v = 0; v < kMaxVariants; ++v) ms[p * kMaxVariants + v] = tune_ms_[p][v];
         }
     }
 
@@ -455,20 +457,21 @@ public:
     }
 
 private:
-    // Tile-width autotuner: each variant of the tuned passes is launched on
-    // this group's (zeroed) work buffer, one warm-up and three timed launches
-    // between HIP events, and the faster is kept. Results are cached per
-    // (precision, n, members) for the process; SPH_AUTOTUNE=0 keeps the
-    // default widths, SPH_TILE_VARIANT=v forces variant v everywhere.
+    // Autotuner: every variant of each tuned pass (tile width, and for the
+    // x pass also the kernel form) is launched on this group's zeroed work
+    // buffer, one warm-up and three timed launches between HIP events, and
+    // the fastest is kept. Results are cached per (precision, n, members) for
+    // the process; SPH_AUTOTUNE=0 keeps variant 0 everywhere,
+    // SPH_TILE_VARIANT=v forces variant v wherever it exists.
     void autotune() {
         using Key = std::tuple<int, int, int>;
         struct Choice {
             int var[kTuned];
-            double ms[kTuned][kTileVariants];
+            double ms[kTuned][kMaxVariants];
         };
         const char* force = std::getenv("SPH_TILE_VARIANT");
-        if (force && force[0] >= '0' && force[0] < '0' + kTileVariants && force[1] == '\0') {
-            for (int p = 0; p < kTuned; ++p) var_[p] = force[0] - '0';
+        if (force && force[0] >= '0' && force[0] < '0' + kMaxVariants && force[1] == '\0') {
+            for (int p = 0; p < kTuned; ++p) var_[p] = std::min(force[0] - '0', pt_->variants[p] - 1);
             return;
         }
         static std::mutex mu;
@@ -480,7 +483,7 @@ private:
             if (it != cache.end()) {
                 for (int p = 0; p < kTuned; ++p) {
                     var_[p] = it->second.var[p];
-                    for (int v = 0; v < kTileVariants; ++v) tune_ms_[p][v] = it->second.ms[p][v];
+                    for (int v = 0; v < kMaxVariants; ++v) tune_ms_[p][v] = it->second.ms[p][v];
                 }
                 return;
             }
@@ -494,7 +497,8 @@ private:
         hip_check(hipEventCreate(&b), "hipEventCreate");
         const int pass_of[kTuned] = {1, 2, 3};
         for (int p = 0; p < kTuned; ++p) {
-            for (int v = 0; v < kTileVariants; ++v) {
+            int best = 0;
+            for (int v = 0; v < pt_->variants[p]; ++v) {
                 var_[p] = v;
                 launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
                 hip_check(hipEventRecord(a, s), "hipEventRecord");
@@ -504,8 +508,9 @@ private:
                 float e = 0.f;
                 hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
                 tune_ms_[p][v] = e / 3.0;
+                if (tune_ms_[p][v] < tune_ms_[p][best]) best = v;
             }
-            var_[p] = tune_ms_[p][1] < tune_ms_[p][0] ? 1 : 0;
+            var_[p] = best;
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
@@ -515,7 +520,7 @@ private:
         Choice c{};
         for (int p = 0; p < kTuned; ++p) {
             c.var[p] = var_[p];
-            for (int v = 0; v < kTileVariants; ++v) c.ms[p][v] = tune_ms_[p][v];
+            for (int v = 0; v < kMaxVariants; ++v) c.ms[p][v] = tune_ms_[p][v];
         }
         std::lock_guard<std::mutex> lock(mu);
         cache.emplace(key, c);
@@ -535,8 +540,8 @@ private:
     DeviceArray<double> partial_, result_;
     std::vector<T> nu_;
     std::vector<double> last_dt_;
-    int var_[kTuned] = {0, 0, 0};  // tile variant of each tuned pass
-    double tune_ms_[kTuned][kTileVariants] = {};
+    int var_[kTuned] = {0, 0, 0};  // variant of each tuned pass
+    double tune_ms_[kTuned][kMaxVariants] = {};
 };
 
 // Per-step GPU time from event pairs, resolved lazily.
@@ -623,7 +628,7 @@ public:
     virtual void set_graph(bool on) = 0;
     virtual void synchronize() = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
-    virtual void tuning(int* lines, double* ms) const = 0;
+    virtual void tuning(int* variant, int* lines, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
 };
@@ -687,7 +692,7 @@ public:
     void set_graph(bool on) override { use_graph_ = on; }
     void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }
     void time_passes(int reps, double* ms, double* bytes) override { g_.time_passes(reps, ms, bytes, stream_.get()); }
-    void tuning(int* lines, double* ms) const override { g_.tuning(lines, ms); }
+    void tuning(int* variant, int* lines, double* ms) const override { g_.tuning(variant, lines, ms); }
 
 private:
     Stream stream_;

@@ -183,17 +183,19 @@ def test_cli_fixed_dt(tmp_path):
 
 
 def test_tile_variants_agree_bitwise_and_match_oracle(monkeypatch):
-    """Both tile widths of the y and x passes (the autotuner's choices) give
-    the same bits, and the oracle's state at 1e-12."""
+    """Every autotuner variant of the y and x passes (tile widths, and the
+    register-direct and LDS-staged x pass) gives the oracle's state at 1e-12
+    and the others' to rounding (the compiler contracts multiply-adds
+    differently in different kernels, so the bits may differ)."""
     n, steps = 32, 3
     o, u = random_state(n, 11)
     states = []
-    for v in ("0", "1"):
+    for v in ("0", "1", "2"):
         print("case tile variant", v, flush=True)
         monkeypatch.setenv("SPH_TILE_VARIANT", v)
         g = sph.TaylorGreenSolver(n, 400.0, "fp64")
-        # FP64 below n = 1024: 8-line tiles, the narrow variant 4 lines
-        assert [t[0] for t in g.tuning().values()] == [{"0": 8, "1": 4}[v]] * 3
+        k = int(v)
+        assert [t[0] for t in g.tuning().values()] == [min(k, 1), k, min(k, 1)]
         for c in range(3):
             g.set_u_hat(c, oracle.c2c_to_half(u[c]))
         for _ in range(steps):
@@ -202,8 +204,10 @@ def test_tile_variants_agree_bitwise_and_match_oracle(monkeypatch):
         g.close()
     for _ in range(steps):
         o.step_rk3(2e-3)
-    for c in range(3):
-        assert np.array_equal(states[0][c], states[1][c])
-    num = sum(np.linalg.norm(states[1][c] - oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
+    for st in states[1:]:
+        num = sum(np.linalg.norm(st[c] - states[0][c]) ** 2 for c in range(3))
+        den = sum(np.linalg.norm(states[0][c]) ** 2 for c in range(3))
+        assert np.sqrt(num / den) < 1e-14
+    num = sum(np.linalg.norm(states[0][c] - oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
     den = sum(np.linalg.norm(oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
     assert np.sqrt(num / den) < 1e-12
