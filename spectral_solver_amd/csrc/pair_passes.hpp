@@ -529,6 +529,23 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
         const int lo = Z::e_last(tid, h / RL, h % RL), hi = Z::e_last(tid, (h + HS) / RL, (h + HS) % RL);
         return f ? hi : lo;
     };
+    // per owned mode: signed kz, |k|^2, mask and the conjugate phase, looked up
+    // once (not per product pair), and the stage coefficients, so that no
+    // dependent global load sits between a transform and its accumulation
+    int mzh[HS], k2h[HS];
+    bool keeph[HS];
+    cx<T> phc[HS];
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+        mzh[h] = signed_mode(owned_e(h), a.n);
+        k2h[h] = kx * kx + my * my + mzh[h] * mzh[h];
+        keeph[h] = keep_mode(k2h[h], a.n);
+        const cx<T> ph = a.phase[kx + my + mzh[h] + a.n];
+        phc[h] = {ph.x, -ph.y};
+    }
+    const T alpha = coef[2 * a.stage];
+    const T beta = coef[2 * a.stage + 1];
+    const T nu = coef[6];
     cx<T> r0[HS], r1[HS], r2[HS];
 #pragma unroll
     for (int h = 0; h < HS; ++h) r0[h] = r1[h] = r2[h] = czero<T>();
@@ -561,63 +578,62 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
             const cx<T> other = shfl_xor1(f ? lo : hi);
             const cx<T> P = f ? other : mine;
             const cx<T> Q = f ? mine : other;
-            const int mz = signed_mode(owned_e(h), a.n);
-            const int k2 = kx * kx + my * my + mz * mz;
-            const cx<T> ph = a.phase[kx + my + mz + a.n];
-            cx<T> val = (P + cmul(Q, cx<T>{ph.x, -ph.y})) * T(0.5);
-            if (!keep_mode(k2, a.n)) val = czero<T>();
-            const T fz = T(mz);
+            cx<T> val = (P + cmul(Q, phc[h])) * T(0.5);
+            if (!keeph[h]) val = czero<T>();
+            const T fz = T(mzh[h]);
             const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
             const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
             const cx<T> da = {kb * val.y, -kb * val.x};  // -i k_b v into r_a
             const cx<T> db = {ka * val.y, -ka * val.x};  // -i k_a v into r_b
-            if (pa == 0)
-                r0[h] = r0[h] + da;
-            else if (pa == 1)
-                r1[h] = r1[h] + da;
-            else
-                r2[h] = r2[h] + da;
-            if (pa != pb) {
-                if (pb == 1)
-                    r1[h] = r1[h] + db;
-                else
-                    r2[h] = r2[h] + db;
-            }
+            // branch-free: at most one of the two terms of each sum is
+            // nonzero, so every accumulator receives exactly the terms (and
+            // roundings) of the reference's order
+            r0[h] = r0[h] + (pa == 0 ? da : czero<T>());
+            r1[h] = r1[h] + ((pa == 1 ? da : czero<T>()) + (pb == 1 && pa != 1 ? db : czero<T>()));
+            r2[h] = r2[h] + ((pa == 2 ? da : czero<T>()) + (pb == 2 && pa != 2 ? db : czero<T>()));
         }
     }
     if (!ok) return;
-    const T alpha = coef[2 * a.stage];
-    const T beta = coef[2 * a.stage + 1];
-    const T nu = coef[6];
     const bool first = a.stage == 0;
     const Buf<T> U(u3 + plane * 3, N * ny * 3 * CB);
     const Buf<T> G(g3 + plane * 3, N * ny * 3 * CB);
+    // all loads of the update first (the buffer accesses may alias as far as
+    // the compiler knows, so loads placed after stores would wait for them)
+    uint32_t off[HS];
+    cx<T> u[HS][3], g[HS][3];
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
-        const int e = owned_e(h);
-        const uint32_t off = (e * ny + kyc) * 3 * CB;
-        const int mz = signed_mode(e, a.n);
-        const int k2 = kx * kx + my * my + mz * mz;
-        const bool keep = keep_mode(k2, a.n);
-        cx<T> u[3] = {U.ld(off), U.ld(off + CB), U.ld(off + 2 * CB)};
+        off[h] = (owned_e(h) * ny + kyc) * 3 * CB;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[h][c] = U.ld(off[h] + c * CB);
+    }
+#pragma unroll
+    for (int h = 0; h < HS; ++h)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[h][c] = first ? czero<T>() : G.ld(off[h] + c * CB);
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+        const int mz = mzh[h], k2 = k2h[h];
+        const bool keep = keeph[h];
         cx<T> rr[3] = {r0[h], r1[h], r2[h]};
         const T vk2 = nu * T(k2);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[c] * vk2;
+        for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[h][c] * vk2;
         project_or_zero(rr[0], rr[1], rr[2], kx, my, mz, k2, keep);
-        cx<T> g[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            g[c] = first ? rr[c] : G.ld(off + c * CB) * alpha + rr[c];
-            u[c] = u[c] + g[c] * beta;
+            g[h][c] = first ? rr[c] : g[h][c] * alpha + rr[c];
+            u[h][c] = u[h][c] + g[h][c] * beta;
         }
-        project_or_zero(u[0], u[1], u[2], kx, my, mz, k2, keep);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            G.st(off + c * CB, g[c]);
-            U.st(off + c * CB, u[c]);
-        }
+        project_or_zero(u[h][0], u[h][1], u[h][2], kx, my, mz, k2, keep);
     }
+#pragma unroll
+    for (int h = 0; h < HS; ++h)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G.st(off[h] + c * CB, g[h][c]);
+            U.st(off[h] + c * CB, u[h][c]);
+        }
 }
 
 // max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)
