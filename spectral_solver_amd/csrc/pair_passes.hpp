@@ -79,6 +79,23 @@ struct alignas(4 * sizeof(T)) cx2 {
     cx<T> b;
 };
 
+// cx2 through a raw buffer resource (32-bit offset; reads past the resource's
+// extent return zeros)
+template <typename T>
+__device__ __forceinline__ cx2<T> ld_cx2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    using V4 = int __attribute__((ext_vector_type(4)));
+    cx2<T> q;
+    if constexpr (sizeof(T) == 4) {
+        q = __builtin_bit_cast(cx2<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    } else {
+        const V4 lo = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        const V4 hi = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0);
+        q.a = __builtin_bit_cast(cx<T>, lo);
+        q.b = __builtin_bit_cast(cx<T>, hi);
+    }
+    return q;
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
     return {__shfl_xor(v.x, 1), __shfl_xor(v.y, 1)};
@@ -244,13 +261,11 @@ __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, i
         const int idx = tid + i * NT;
         if (LOADS % NT == 0 || idx < LOADS) {
             const int l = idx % NL, k = idx / NL;
-            cx<T> va = czero<T>(), vb = czero<T>();
-            if (y0 + l < ny) {
-                const size_t off = (static_cast<size_t>(k) * nz + z) * ny + y0 + l;
-                const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
-                va = q.a;
-                vb = q.b;
-            }
+            // lines past the last one read a valid row and discard it
+            const bool ok = y0 + l < ny;
+            const size_t off = (static_cast<size_t>(k) * nz + z) * ny + (ok ? y0 + l : ny - 1);
+            const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
+            cx<T> va = ok ? q.a : czero<T>(), vb = ok ? q.b : czero<T>();
             if (k == 0 || k == N / 2) {
                 va.y = T(0);
                 vb.y = T(0);
@@ -372,7 +387,10 @@ struct PairXArgs {
 // of Z_k and Z_{N-k}. Same formulas as the LDS-staged form below (results
 // agree to rounding; the compiler may contract multiply-adds differently).
 template <typename T, int N, int LINES = 0>
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT)) k_px_products_rd(PairXArgs<T> a) {
+// 512-thread tiles ask for two work-groups per CU (16 waves), which caps the
+// registers at 128 like the 1024-thread form.
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 2 : 1))
+    k_px_products_rd(PairXArgs<T> a) {
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
     constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
@@ -384,27 +402,45 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT)) k_px_products_rd(Pai
     const int z = blockIdx.y;
     const bool ok = y0 + l < a.ny;
     const long long zw = blockIdx.z * a.ms_w;
-    const size_t row = static_cast<size_t>(a.nz) * a.ny;  // cx2 elements between k rows
-    const size_t lane = static_cast<size_t>(z) * a.ny + (ok ? y0 + l : 0);
+    const uint32_t row = static_cast<uint32_t>(a.nz) * a.ny;  // cx2 elements between k rows
+    // per-slot 32-bit addressing fits when STEP rows of a cubic grid span
+    // less than 2 GiB (everything but FP64 at N = 1024); the launcher only
+    // runs this form with ny * nz <= N * N
+    constexpr bool kNarrow = static_cast<long long>(Z::STEP) * N * N * sizeof(cx2<T>) < (1LL << 31);
     load_twiddles<T, N, NT>(twl, a.tw, tid);
     // u, s at slot m of element j + m*N/E (j = tid / NL)
     T u[3][E], s[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw);
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + static_cast<size_t>(z) * a.ny + y0;
         cx<T> v[1][T0][R0], w[1][TLS][RL];
 #pragma unroll
         for (int i = 0; i < T0; ++i)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
+                // slot m holds element e = j + m*STEP: over the work-group the
+                // rows k = min(e, N - e) span at most STEP rows from kmin, so
+                // one resource per slot (based at row kmin) and a 32-bit lane
+                // offset address them; lanes past the last line read zeros
+                constexpr int STEP = Z::STEP;
+                const int m = i + r * T0;
+                const int lo = m * STEP, hi = (m + 1) * STEP - 1;
+                const int kmin = hi <= N / 2 ? lo : (lo > N / 2 ? N - hi : (lo < N - hi ? lo : N - hi));
                 const int e = Z::e_first(tid, i, r);
                 const int k = e <= N / 2 ? e : N - e;
-                cx<T> va = czero<T>(), vb = czero<T>();
-                if (ok) {
-                    const cx2<T> q = AB[k * row + lane];
-                    va = q.a;
-                    vb = q.b;
+                cx2<T> q;
+                if constexpr (kNarrow) {
+                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<cx2<T>*>(AB + static_cast<size_t>(kmin) * row), 0,
+                        static_cast<int>(STEP * row * sizeof(cx2<T>)), 0x00020000);
+                    const uint32_t off =
+                        ok ? ((k - kmin) * row + l) * static_cast<uint32_t>(sizeof(cx2<T>)) : 0xFFFFFFC0u;
+                    q = ld_cx2<T>(rs, off);
+                } else {
+                    q = AB[static_cast<size_t>(k) * row + (ok ? l : 0)];
+                    if (!ok) q = cx2<T>{};
                 }
+                cx<T> va = q.a, vb = q.b;
                 if (k == 0 || k == N / 2) {
                     va.y = T(0);
                     vb.y = T(0);

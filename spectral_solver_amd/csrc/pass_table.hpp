@@ -128,6 +128,8 @@ struct Launch {
         k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
     }
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
+        // the register-direct form addresses rows of at most N * N modes
+        if (static_cast<long long>(ny) * nz > static_cast<long long>(N) * N) variant = 2;
         if (variant == 2) {
             const dim3 grid((ny + L0 - 1) / L0, nz, members);
             k_px_products<T, N, L0><<<grid, TL::NT, 0, s>>>(a);
