@@ -10,6 +10,9 @@
 - spectral_Re1600_512_head.dat: the first rows of the literature Taylor-Green
   curve shipped with the reference (spectral_Re1600_512.dat: t, E, eps,
   enstrophy), copied as data.
+- spectral_Re1600_512.dat: that whole curve (t = 0 .. 19.99 every 0.01), the
+  data the reference's comparison script compare_taylor_green.py plots
+  against, copied as data.
 """
 import os
 import sys
@@ -47,17 +50,27 @@ def tg16():
                         diagnostics=np.array(diags), **out)
 
 
+DNS = "/root/reference/spectral_Re1600_512.dat"
+
+
 def dns_head(rows=5):
-    src = "/root/reference/spectral_Re1600_512.dat"
-    with open(src) as f:
+    with open(DNS) as f:
         lines = [ln for ln in f if ln.strip()][:rows]
     with open(os.path.join(HERE, "spectral_Re1600_512_head.dat"), "w") as f:
+        f.writelines(lines)
+
+
+def dns_full():
+    with open(DNS) as f:
+        lines = [ln for ln in f if ln.strip()]
+    with open(os.path.join(HERE, "spectral_Re1600_512.dat"), "w") as f:
         f.writelines(lines)
 
 
 if __name__ == "__main__":
     fft_vectors()
     tg16()
-    if os.path.exists("/root/reference/spectral_Re1600_512.dat"):
+    if os.path.exists(DNS):
         dns_head()
+        dns_full()
     print("fixtures written to", HERE)
