@@ -1,5 +1,5 @@
 """Writes tests/golden/tg256_fp64_500.npz with the CPU oracle (run in the build
-container; about 1.5 h on 6 cores).
+container; 2.5 h at low priority on 6 cores).
 
 Taylor-Green 256^3, nu = 1/1600, dt = 1e-3, 500 fixed RK3 steps, FP64. Stored:
 per-step kinetic energy and dissipation (steps 0..500), and at step 500 the
