@@ -150,13 +150,16 @@ int sph_solver_set_graph(sph_solver_t solver, int enabled);
 /* Wait for all work queued by the solver. */
 int sph_solver_synchronize(sph_solver_t solver);
 
-/* Measure the pass kernels of one RK3 stage with HIP events on the solver's
- * stream: each of the five passes is launched `reps` times back to back on
- * the current state (the RK-update pass on a scratch copy of the state) and
- * its mean duration in milliseconds is written to ms[5] in the order
- * inverse-z, inverse-y, x (inverse x + products + forward x), forward-y,
- * forward-z(+update). Also returns the algorithmic HBM bytes per launch of
- * each pass in bytes[5]. The state is unchanged. */
+/* Measure the pass kernels of an RK3 stage with HIP events on the solver's
+ * stream: each of the five pass kernels is launched `reps` times back to
+ * back on the current state (the RK-update pass on a scratch copy of the
+ * state) and its mean duration in milliseconds is written to ms[5] in the
+ * order inverse-z, inverse-y, x (inverse x + products + forward x),
+ * forward-y, forward-z + update + the next stage's inverse z. A stage runs
+ * the last four; the standalone inverse z runs only before a step whose
+ * state was set from outside (creation, set_uhat). Also returns the
+ * algorithmic HBM bytes per launch of each pass in bytes[5]. The state is
+ * unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
 /* The choices of the solver's autotuner (the reference plans its FFTs with

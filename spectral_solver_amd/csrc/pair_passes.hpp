@@ -102,17 +102,16 @@ __device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
 }
 
 // ---------------------------------------------------------------------------
-// inverse z: state -> pair fields
+// inverse z: state -> pair fields. One tile: the body below runs in
+// k_pinv_z and, for the next RK stage, at the end of k_pfwd_z_rk_inv; the
+// twiddles are in twl already.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a) {
+__device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
+                                            int tid) {
     using Z = ZTile<T, N>;
-    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
-    __shared__ cx<T> lds[Z::LDS];
-    __shared__ cx<T> twl[N];
-    const int tid = threadIdx.x;
-    const TileId tile = xcd_tile_id();
     const int l = tid % NL, f = l & 1;
     const int ky = tile.x * (NL / 2) + (l >> 1);
     const int kx = tile.y;
@@ -128,7 +127,6 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
     const uint32_t u_lane = ((tid / NL) * ny + kyc) * 3 * CB, u_row = ny * 3 * CB;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
-    load_twiddles<T, N, NT>(twl, a.tw, tid);
     cx<T> v[1][T0][R0], w[1][TLS][RL];
     auto uoff = [&](int i, int r, int c) -> uint32_t {
         return u_lane + (i * Z::STEP + r * (N / R0)) * u_row + c * CB;
@@ -206,6 +204,16 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a)
             store(a.out[q]);
         }
     }
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a) {
+    using Z = ZTile<T, N>;
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
+    pinv_z_tile<T, N, MODE>(a, xcd_tile_id(), lds, twl, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,19 +538,17 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
 }
 
 // ---------------------------------------------------------------------------
-// forward z of the six product pairs + RHS completion + RK3 update
+// forward z of the six product pairs + RHS completion + RK3 update (one tile;
+// the twiddles are in twl already)
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T> a) {
+__device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
+                                               int tid) {
     using Z = ZTile<T, N>;
-    constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
+    constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     constexpr int SL = TLS * RL;  // last-stage slots per lane
     constexpr int HS = SL / 2;    // modes owned by each lane of a line
     static_assert(SL % 2 == 0, "the two lanes of a line split the slots");
-    __shared__ cx<T> lds[Z::LDS];
-    __shared__ cx<T> twl[N];
-    const int tid = threadIdx.x;
-    const TileId tile = xcd_tile_id();
     const int l = tid % NL, f = l & 1;
     const int ky = tile.x * (NL / 2) + (l >> 1);
     const int kx = tile.y;
@@ -557,7 +563,6 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
     const size_t plane = static_cast<size_t>(kx) * N * ny;
     const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
-    load_twiddles<T, N, NT>(twl, a.tw, tid);
     const int my = signed_mode(kyc, a.n);
     const T fx = T(kx), fy = T(my);
     // element (kz) of owned mode h of this lane: slot h (f = 0) or h + HS (f = 1)
@@ -670,6 +675,31 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk(PairFwdZArgs<T>
             G.st(off[h] + c * CB, g[h][c]);
             U.st(off[h] + c * CB, u[h][c]);
         }
+}
+
+// The last pass of an RK stage: forward z + RHS completion + update of the
+// tile's (kx, ky) lines, then the next stage's inverse z of the same lines
+// (impl.hpp:979-1045 runs compute_rhs's backward transforms of u_hat at the
+// start of every stage; here they follow the update that produced u_hat, so
+// the state is read back from L2 rather than from HBM and one launch per
+// stage goes away). The inverse half writes the velocity pairs into
+// inv.out[0..2] (the first three product pair fields, whose values the
+// forward half has consumed: every work-group reads its own lines only).
+template <typename T, int N>
+__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArgs<T> a, PairInvZArgs<T> inv) {
+    using Z = ZTile<T, N>;
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
+    load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
+    pfwd_z_rk_tile<T, N>(a, tile, lds, twl, tid);
+    // work-group scope is enough for the read-back: the barrier's release /
+    // acquire fences make every wave's state stores visible to the others
+    // (one CU, one L1). A device-scope fence here would write back the XCD's
+    // whole L2 per work-group (measured: 6x slower pass).
+    __syncthreads();
+    pinv_z_tile<T, N, PZ_RHS>(inv, tile, lds, twl, tid);
 }
 
 // max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)

@@ -33,7 +33,8 @@ struct PassTable {
                    const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
     void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, hipStream_t);
-    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, int nky, int nkx, int members, hipStream_t);
+    // forward z + update, then the next stage's inverse z (k_pfwd_z_rk_inv)
+    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -139,9 +140,10 @@ struct Launch {
             px_products_rd<X0>(a, ny, nz, members, s);
         }
     }
-    static void pfwd_z_rk(const PairFwdZArgs<T>& a, int nky, int nkx, int members, hipStream_t s) {
+    static void pfwd_z_rk(const PairFwdZArgs<T>& a, const PairInvZArgs<T>& inv, int nky, int nkx, int members,
+                          hipStream_t s) {
         const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
-        k_pfwd_z_rk<T, N><<<grid, ZT::NT, 0, s>>>(a);
+        k_pfwd_z_rk_inv<T, N><<<grid, ZT::NT, 0, s>>>(a, inv);
     }
     static constexpr PassTable<T> table{N,
                                         TL::NL,

@@ -193,7 +193,8 @@ public:
         }
     }
 
-    static constexpr int kPasses = 5;  // kernel launches per RK stage
+    // pass kernels: 0 (inverse z) runs only in prepare_pairs, 1-4 per stage
+    static constexpr int kPasses = 5;
     int n = 0, count = 0, H = 0;
     std::int64_t modes = 0, npts = 0, su = 0, sw = 0;  // per-member strides: state, work
 
@@ -216,28 +217,35 @@ public:
         hip_check(hipGetLastError(), "coefficient upload");
     }
 
+    // Inverse z of the state u (member 0 of the launch) into the velocity
+    // pairs, the first three pair fields of W.
+    PairInvZArgs<T> inv_z_args(const cx<T>* u, cx<T>* W) const {
+        PairInvZArgs<T> a{};
+        a.u3 = u;
+        for (int c = 0; c < 3; ++c) a.out[c] = W + c * 2 * modes;
+        a.ny = n;
+        a.n = n;
+        a.phase = phase_.get();
+        a.tw = tw_.get();
+        a.ms_u = su;
+        a.ms_w = sw;
+        return a;
+    }
+
     // pass k (0 inverse z, 1 inverse y, 2 inverse x + products + forward x,
-    // 3 forward y, 4 forward z + update) of stage `stage` for members
-    // [first, first + members); u and g address member 0 of the state family.
+    // 3 forward y, 4 forward z + update + the next stage's inverse z) of
+    // stage `stage` for members [first, first + members); u and g address
+    // member 0 of the state family. Pass 0 runs only when the velocity pairs
+    // in W are not those of the state (pairs_valid()); pass 4 leaves them so.
     void launch_pass(int k, int stage, cx<T>* u, cx<T>* g, int first, int members, hipStream_t s) const {
         const std::int64_t M2 = 2 * modes;
         cx<T>* W = W_.get() + first * sw;
         u += first * su;
         g += first * su;
         switch (k) {
-            case 0: {
-                PairInvZArgs<T> a{};
-                a.u3 = u;
-                for (int c = 0; c < 3; ++c) a.out[c] = W + c * M2;
-                a.ny = n;
-                a.n = n;
-                a.phase = phase_.get();
-                a.tw = tw_.get();
-                a.ms_u = su;
-                a.ms_w = sw;
-                pt_->pinv_z(a, PZ_RHS, n, H, members, s);
+            case 0:
+                pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, s);
                 break;
-            }
             case 1:
                 pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
@@ -269,21 +277,36 @@ public:
                 a.tw = tw_.get();
                 a.ms_u = su;
                 a.ms_w = sw;
-                pt_->pfwd_z_rk(a, n, H, members, s);
+                pt_->pfwd_z_rk(a, inv_z_args(u, W), n, H, members, s);
                 break;
             }
         }
     }
 
+    // Velocity pairs of the current state for the first stage: the last pass
+    // of every stage leaves them in W, so they are recomputed only after the
+    // state was set from outside (initial condition, set_uhat) or W was used
+    // for something else (time_passes).
+    void prepare_pairs(hipStream_t s) {
+        if (pairs_valid_) return;
+        launch_pass(0, 0, U_.get(), G_.get(), 0, count, s);
+        pairs_valid_ = true;
+    }
+    void invalidate_pairs() { pairs_valid_ = false; }
+    bool pairs_valid() const { return pairs_valid_; }
+
+    // passes 1-4 of a stage (prepare_pairs first)
     void launch_stage(int stage, hipStream_t s) const {
-        for (int k = 0; k < kPasses; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
+        for (int k = 1; k < kPasses; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
     }
 
     // member m: u_hat -> physical (the group's one physical buffer).
-    // PZ_DIAG: u in P[0..2], omega in P[3..5]; PZ_VEL: u in P[0..2].
+    // PZ_DIAG: u in P[0..2], omega in P[3..5]; PZ_VEL: u in P[0..2]. The
+    // transforms run in the back half of W (product pair fields 3-5, unused
+    // between stages), so the velocity pairs for the next step stay intact.
     void inverse(int m, int mode, hipStream_t s) const {
         const std::int64_t M2 = 2 * modes;
-        cx<T>* W = W_.get() + m * sw;
+        cx<T>* W = W_.get() + m * sw + 3 * M2;
         T* P = P_.get();
         PairInvZArgs<T> a{};
         a.u3 = U_.get() + m * su;
@@ -378,6 +401,7 @@ public:
         hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
                   "hipMemcpy set_uhat");
         hip_check(hipStreamSynchronize(s), "set_uhat");
+        pairs_valid_ = false;
     }
 
     // all three components of member m at once
@@ -389,6 +413,7 @@ public:
         hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
                   "hipMemcpy set_state");
         hip_check(hipStreamSynchronize(s), "set_state");
+        pairs_valid_ = false;
     }
 
     // impl.hpp:796-817 for member m: TG field at cell centres, 3 forward
@@ -417,11 +442,14 @@ public:
         fz.tw = tw_.get();
         pt_->fwd_z(fz, FZ_PROJECT, n, H, s);
         hip_check(hipGetLastError(), "initial condition launch");
+        pairs_valid_ = false;
     }
 
     // Mean duration of each pass kernel (all members) over `reps` launches;
-    // the RK pass updates scratch copies of the state so the state is kept.
+    // the RK pass updates scratch copies of the state so the state is kept
+    // (the work buffer is not: the step after this recomputes its pairs).
     void time_passes(int reps, double* ms, double* bytes, hipStream_t s) {
+        pairs_valid_ = false;
         if (reps < 1) reps = 1;
         hip_check(hipStreamSynchronize(s), "time_passes");
         DeviceArray<cx<T>> u2(count * su), g2(count * su);
@@ -449,9 +477,11 @@ public:
         (void)hipEventDestroy(b);
         hip_check(hipGetLastError(), "time_passes");
         // algorithmic bytes (each array read or written once), C = one
-        // complex half-spectrum field
+        // complex half-spectrum field; the last pass reads 12C + u, g and
+        // writes u, g and the next stage's 6C of velocity pairs (its state
+        // read-back comes from L2)
         const double C = static_cast<double>(modes) * 2.0 * sizeof(T) * count;
-        const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 24 * C};
+        const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 30 * C};
         for (int k = 0; k < kPasses; ++k) bytes[k] = b5[k];
         hip_check(hipStreamSynchronize(s), "time_passes");
     }
@@ -540,6 +570,7 @@ private:
     DeviceArray<double> partial_, result_;
     std::vector<T> nu_;
     std::vector<double> last_dt_;
+    bool pairs_valid_ = false;     // W holds the velocity pairs of U (every member)
     int var_[kTuned] = {0, 0, 0};  // variant of each tuned pass
     double tune_ms_[kTuned][kMaxVariants] = {};
 };
@@ -649,6 +680,7 @@ public:
         const hipStream_t s = stream_.get();
         g_.set_dt(&dt, s);
         auto ev = timer_.begin(s);
+        g_.prepare_pairs(s);
         if (use_graph_) {
             graph_.launch(s, [&] {
                 for (int st = 0; st < 3; ++st) g_.launch_stage(st, s);
@@ -765,7 +797,8 @@ public:
         for (std::size_t j = 0; j < where_.size(); ++j) dt_[where_[j].first][where_[j].second] = dt[j];
         for (std::size_t q = 0; q < groups_.size(); ++q) groups_[q]->set_dt(dt_[q].data(), s);
         auto ev = timer_.begin(s);
-        // one branch per size: the groups' chains of 15 launches are
+        for (auto& g : groups_) g->prepare_pairs(s);
+        // one branch per size: the groups' chains of 12 launches are
         // independent, so small grids run beside large ones
         auto record = [&] {
             if (groups_.size() == 1) {
