@@ -1,9 +1,10 @@
 // Device building blocks of the batched 1D FFT used by every pass kernel.
 //
 // A work-group owns a "tile" of NL lines of length N held in LDS with layout
-// [element][line] (line fastest, row stride NL+1 complex values so that both
-// the line-fast stage accesses and the element-fast global staging are free
-// of systematic bank conflicts). Each Stockham stage of radix R reads R values
+// [element][line] (line fastest; rows padded to NL+1 complex values so that
+// both the line-fast stage accesses and the element-fast global staging are
+// free of systematic bank conflicts, or, for narrow FP32 tiles, unpadded and
+// rotated: tile_at, ztile_stride). Each Stockham stage of radix R reads R values
 // per butterfly task into registers, applies twiddles and a register DFT_R,
 // then writes them back in self-sorting (Stockham) order:
 //   x_r = a[j + r*N/R] * w_{Ns*R}^{r*(j mod Ns)},  y = DFT_R(x),
@@ -201,14 +202,44 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
+// Tile layout in LDS: element `row` of line `line`.
+//   padded:   row * (NL + 1) + line
+//   swizzled: row * NL + ((line + 2 * (row / (16 / NL))) mod NL), FP32 with
+//             NL = 2, 4 or 8 lines only.
+// The LDS serves 64-bit accesses 16 lanes at a time over 32 four-byte banks.
+// An unpadded FP32 row covers 2*NL banks, so rows r and r + 16/NL share
+// banks; the rotation moves the second onto other banks. That makes the
+// contiguous global <-> LDS copies of a pair-line tile (16 lanes: 8
+// consecutive rows x the 2 lines of a pair) conflict-free, and the stage
+// reads (16 lanes: 16/NL consecutive rows x NL lines) stay so. The first
+// stage's writes (rows j*R + r for consecutive j) additionally go out in the
+// order r ^ (j mod 16/NL) (fft_stage).
+// ---------------------------------------------------------------------------
+template <int NL, bool SWZ>
+__device__ __forceinline__ int tile_at(int row, int line) {
+    if constexpr (SWZ) {
+        static_assert(NL == 2 || NL == 4 || NL == 8, "swizzled rows hold 2, 4 or 8 lines");
+        constexpr unsigned RB = 16 / NL;
+        const unsigned r = static_cast<unsigned>(row), l = static_cast<unsigned>(line);
+        return static_cast<int>(r * NL + ((l + 2 * (r / RB)) & (NL - 1)));
+    } else {
+        return row * (NL + 1) + line;
+    }
+}
+
+template <int NL, bool SWZ>
+constexpr int tile_lds(int n) {
+    return SWZ ? NL * n : (NL + 1) * n;
+}
+
+// ---------------------------------------------------------------------------
 // One in-place Stockham stage over the whole tile.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int NL, int NT, int R, int NS, bool INV>
+template <typename T, int N, int NL, int NT, int R, int NS, bool INV, bool SWZ = false>
 __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* twl, int tid) {
     constexpr int M = N / R;
     constexpr int TASKS = NL * M;
     constexpr int TPT = (TASKS + NT - 1) / NT;
-    constexpr int STRIDE = NL + 1;
     cx<T> v[TPT][R];
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
@@ -217,7 +248,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* twl, int tid)
             const int line = t % NL;
             const int j = t / NL;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * M) * STRIDE + line];
+            for (int r = 0; r < R; ++r) v[i][r] = buf[tile_at<NL, SWZ>(j + r * M, line)];
             if constexpr (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
@@ -231,6 +262,10 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* twl, int tid)
         }
     }
     __syncthreads();
+    // first stage of a swizzled tile: lane j writes element r ^ (j & SWM) in
+    // write r (R a power of two)
+    constexpr bool kPow2 = (R & (R - 1)) == 0;
+    constexpr int SWM = SWZ && NS == 1 && kPow2 ? ((16 / NL) < R ? 16 / NL : R) - 1 : 0;
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
         const int t = tid + i * NT;
@@ -239,20 +274,37 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* twl, int tid)
             const int j = t / NL;
             const int k = j % NS;
             const int base = (j - k) * R + k;
+            if constexpr (SWM == 0) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[(base + r * NS) * STRIDE + line] = v[i][r];
+                for (int r = 0; r < R; ++r) buf[tile_at<NL, SWZ>(base + r * NS, line)] = v[i][r];
+            } else {
+                const int sw = j & SWM;
+#pragma unroll
+                for (int b = 1; b <= SWM; b <<= 1) {
+                    const bool take = (sw & b) != 0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (!(r & b)) {
+                            const cx<T> a0 = v[i][r], a1 = v[i][r | b];
+                            v[i][r] = take ? a1 : a0;
+                            v[i][r | b] = take ? a0 : a1;
+                        }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[tile_at<NL, SWZ>(base + (r ^ sw) * NS, line)] = v[i][r];
+            }
         }
     }
     __syncthreads();
 }
 
-template <typename T, int N, int NL, int NT, bool INV, int S = 0, int NS = 1>
+template <typename T, int N, int NL, int NT, bool INV, int S = 0, int NS = 1, bool SWZ = false>
 __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) {
     constexpr RadixPlan P = radix_plan(N);
     if constexpr (S < P.count) {
         constexpr int R = P.r[S];
-        fft_stage<T, N, NL, NT, R, NS, INV>(buf, twl, tid);
-        fft_tile<T, N, NL, NT, INV, S + 1, NS * R>(buf, twl, tid);
+        fft_stage<T, N, NL, NT, R, NS, INV, SWZ>(buf, twl, tid);
+        fft_tile<T, N, NL, NT, INV, S + 1, NS * R, SWZ>(buf, twl, tid);
     }
 }
 
@@ -266,7 +318,23 @@ __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) 
 // stage runs in registers, the middle stages go through LDS, and the last
 // stage leaves element j + r*N/RL of line l in registers for the caller's
 // post-operation and coalesced store.
+//
+// LDS rows (one element of every line) and bank conflicts. The LDS serves
+// 64-bit accesses 16 lanes at a time over 32 four-byte banks. An FP32 row of
+// NL <= 8 lines covers 2*NL banks, so 16/NL consecutive rows fill the banks
+// once: stored unpadded, every access whose 16 lanes touch 16/NL consecutive
+// rows is conflict-free (the middle and last stages, and the x pass's own
+// accesses). The first-stage write is the exception: its lanes write rows
+// j*R0 + r for consecutive j, R0 apart, which share banks; there lane j writes
+// its values in the order r ^ (j mod 16/NL), so that a write instruction's
+// rows differ by R0 +- a few and land in distinct banks. Wider FP32 rows and
+// FP64 rows keep one element of padding per row.
 // ---------------------------------------------------------------------------
+template <typename T>
+constexpr int ztile_stride(int nl) {
+    return sizeof(T) == 4 && nl <= 8 ? nl : nl + 1;
+}
+
 template <typename T, int N, int NL_ = 0>  // NL_ = 0: the default width
 struct ZTile {
     static constexpr RadixPlan P = z_radix_plan(N);
@@ -292,7 +360,7 @@ struct ZTile {
     static constexpr int NL = NL_ ? NL_ : pick_nl();
     static constexpr int E = lcm_radix();  // elements per thread and field
     static constexpr int NT = NL * N / E;
-    static constexpr int S = NL + 1;           // LDS row stride
+    static constexpr int S = ztile_stride<T>(NL);  // LDS row stride
     static constexpr int LDS = S * N;          // complex values per field tile
     static constexpr int R0 = P.r[0];
     static constexpr int RL = P.r[P.count - 1];
@@ -314,7 +382,7 @@ __device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid
     constexpr int TASKS = NL * M;
     static_assert(TASKS % NT == 0, "middle stage must split evenly");
     constexpr int TPT = TASKS / NT;
-    constexpr int S = NL + 1;
+    constexpr int S = ztile_stride<T>(NL);
     cx<T> v[NF][TPT][R];
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
@@ -382,13 +450,38 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
 #pragma unroll
             for (int i = 0; i < Z::T0; ++i) dft<R0, INV>(v[f][i]);
         __syncthreads();  // previous readers of the tile buffers are done
+        // unpadded rows: lane j writes element r ^ (j & SWM) in write r
+        constexpr bool kPow2 = (R0 & (R0 - 1)) == 0;
+        constexpr int SWM = S == NL && kPow2 ? (16 / NL < R0 ? 16 / NL : R0) - 1 : 0;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int i = 0; i < Z::T0; ++i) {
                 const int j = tid / NL + i * Z::STEP;
+                cx<T>* const row0 = lds + f * Z::LDS + j * R0 * S + l;
+                if constexpr (SWM == 0) {
 #pragma unroll
-                for (int r = 0; r < R0; ++r) lds[f * Z::LDS + (j * R0 + r) * S + l] = v[f][i][r];
+                    for (int r = 0; r < R0; ++r) row0[r * S] = v[f][i][r];
+                } else {
+                    const int sw = j & SWM;
+                    cx<T> u[R0];
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) u[r] = v[f][i][r];
+                    // u[r] = v[r ^ sw]: one conditional exchange per bit
+#pragma unroll
+                    for (int b = 1; b <= SWM; b <<= 1) {
+                        const bool take = (sw & b) != 0;
+#pragma unroll
+                        for (int r = 0; r < R0; ++r)
+                            if (!(r & b)) {
+                                const cx<T> a0 = u[r], a1 = u[r | b];
+                                u[r] = take ? a1 : a0;
+                                u[r | b] = take ? a0 : a1;
+                            }
+                    }
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) row0[(r ^ sw) * S] = u[r];
+                }
             }
         __syncthreads();
         // stages 1 .. count-2 (NS after stage 0 is R0)

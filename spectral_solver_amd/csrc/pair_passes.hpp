@@ -224,9 +224,11 @@ template <typename T, int N, bool INV, int LINES = default_lines<T, N>()>
 __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     k_plines(cx<T>* data, long long npairlines, long long member_stride, const cx<T>* tw) {
     using TL = Tile<T, N, LINES>;
-    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
-    __shared__ cx<T> buf[TL::LDS];
+    // narrow FP32 tiles: unpadded rotated rows (conflict-free, tile_at)
+    constexpr bool SWZ = sizeof(T) == 4 && NL <= 8;
+    __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const long long g0 = static_cast<long long>(blockIdx.x) * (NL / 2);
@@ -237,15 +239,15 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     for (int i = 0; i < E; ++i) {
         const int m = tid + i * NT;
         const int g = m / (2 * N), rem = m % (2 * N);
-        if (m < TL::ELEMS) buf[(rem >> 1) * S + 2 * g + (rem & 1)] = m < valid ? base[m] : czero<T>();
+        if (m < TL::ELEMS) buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = m < valid ? base[m] : czero<T>();
     }
     __syncthreads();
-    fft_tile<T, N, NL, NT, INV>(buf, twl, tid);
+    fft_tile<T, N, NL, NT, INV, 0, 1, SWZ>(buf, twl, tid);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         const int m = tid + i * NT;
         const int g = m / (2 * N), rem = m % (2 * N);
-        if (m < valid) base[m] = buf[(rem >> 1) * S + 2 * g + (rem & 1)];
+        if (m < valid) base[m] = buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))];
     }
 }
 
