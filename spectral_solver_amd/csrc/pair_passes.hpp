@@ -231,8 +231,9 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
-    const long long g0 = static_cast<long long>(blockIdx.x) * (NL / 2);
-    cx<T>* base = data + blockIdx.y * member_stride + g0 * 2 * N;
+    const TileId tile = xcd_tile_id();  // an XCD's tiles are one contiguous run
+    const long long g0 = static_cast<long long>(tile.x) * (NL / 2);
+    cx<T>* base = data + tile.y * member_stride + g0 * 2 * N;
     const int valid = static_cast<int>(npairlines - g0 < NL / 2 ? npairlines - g0 : NL / 2) * 2 * N;
     load_twiddles<T, N, NT>(twl, tw, tid);
 #pragma unroll
@@ -408,10 +409,13 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int l = tid % NL;
-    const int y0 = blockIdx.x * NL;
-    const int z = blockIdx.y;
+    // consecutive tiles of an XCD: neighbouring y blocks of one z, i.e. the
+    // same translation pages of every k row
+    const TileId tile = xcd_tile_id();
+    const int y0 = tile.x * NL;
+    const int z = tile.y;
     const bool ok = y0 + l < a.ny;
-    const long long zw = blockIdx.z * a.ms_w;
+    const long long zw = tile.z * a.ms_w;
     const uint32_t row = static_cast<uint32_t>(a.nz) * a.ny;  // cx2 elements between k rows
     // per-slot 32-bit addressing fits when STEP rows of a cubic grid span
     // less than 2 GiB (everything but FP64 at N = 1024); the launcher only
@@ -515,9 +519,10 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
     __shared__ cx<T> buf[TL::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
-    const int y0 = blockIdx.x * NL;
-    const int z = blockIdx.y;
-    const long long zw = blockIdx.z * a.ms_w;
+    const TileId tile = xcd_tile_id();  // as in k_px_products_rd
+    const int y0 = tile.x * NL;
+    const int z = tile.y;
+    const long long zw = tile.z * a.ms_w;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
     T u[3][E], s[3][E];
 #pragma unroll
