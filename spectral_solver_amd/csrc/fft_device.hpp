@@ -374,10 +374,11 @@ struct ZTile {
     __device__ static int e_last(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / RL); }
 };
 
-// One in-place middle Stockham stage over NF field tiles (read all, sync,
-// write all, sync).
-template <typename T, int N, int NL, int NT, int NF, int R, int NS, bool INV>
-__device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid) {
+// One middle Stockham stage over NF field tiles. In place (src == dst): read
+// all, sync, write all, sync. Ping-pong (src != dst): read src, write dst,
+// sync; the reads of src and writes of dst need no barrier between them.
+template <typename T, int N, int NL, int NT, int NF, int R, int NS, bool INV, bool INPLACE>
+__device__ __forceinline__ void zstage_mid(const cx<T>* src, cx<T>* dst, const cx<T>* twl, int tid) {
     constexpr int M = N / R;
     constexpr int TASKS = NL * M;
     static_assert(TASKS % NT == 0, "middle stage must split evenly");
@@ -391,7 +392,7 @@ __device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[f][i][r] = lds[f * (S * N) + (j + r * M) * S + line];
+            for (int r = 0; r < R; ++r) v[f][i][r] = src[f * (S * N) + (j + r * M) * S + line];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 cx<T> w = twl[r * k * (N / (NS * R))];
@@ -401,7 +402,7 @@ __device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid
             dft<R, INV>(v[f][i]);
         }
     }
-    __syncthreads();
+    if constexpr (INPLACE) __syncthreads();
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
         const int t = tid + i * NT;
@@ -410,24 +411,42 @@ __device__ __forceinline__ void zstage_mid(cx<T>* lds, const cx<T>* twl, int tid
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[f * (S * N) + (base + r * NS) * S + line] = v[f][i][r];
+            for (int r = 0; r < R; ++r) dst[f * (S * N) + (base + r * NS) * S + line] = v[f][i][r];
     }
     __syncthreads();
 }
 
-// stages S .. count-2; NS is the product of the radices before stage S
-template <typename T, int N, int NL, int NT, int NF, bool INV, int S, int NS>
-__device__ __forceinline__ void zstages_mid(cx<T>* lds, const cx<T>* twl, int tid) {
+// stages S .. count-2; NS is the product of the radices before stage S. The
+// data starts in a; with PP it alternates between a and b. Returns the tile
+// holding the result.
+template <typename T, int N, int NL, int NT, int NF, bool INV, int S, int NS, bool PP>
+__device__ __forceinline__ cx<T>* zstages_mid(cx<T>* a, cx<T>* b, const cx<T>* twl, int tid) {
     constexpr RadixPlan P = z_radix_plan(N);
     if constexpr (S <= P.count - 2) {
-        zstage_mid<T, N, NL, NT, NF, P.r[S], NS, INV>(lds, twl, tid);
-        zstages_mid<T, N, NL, NT, NF, INV, S + 1, NS * P.r[S]>(lds, twl, tid);
+        cx<T>* const dst = PP ? b : a;
+        zstage_mid<T, N, NL, NT, NF, P.r[S], NS, INV, !PP>(a, dst, twl, tid);
+        return zstages_mid<T, N, NL, NT, NF, INV, S + 1, NS * P.r[S], PP>(dst, a, twl, tid);
+    } else {
+        return a;
     }
 }
 
+// Middle stages of the z plan (ping-pong tiles alternate once per middle
+// stage; the first-stage write goes to tile A).
+template <int N>
+constexpr int z_mid_stages() {
+    return z_radix_plan(N).count > 2 ? z_radix_plan(N).count - 2 : 0;
+}
+
+constexpr int kLdsBytes = 160 * 1024;  // LDS per work-group on gfx950
+
 // v: first-stage inputs (consumed), w: last-stage outputs. LINES: tile width
-// (0 = the default ZTile width).
-template <typename T, int N, int NF, bool INV, int LINES = 0, int A0, int B0, int A1, int B1>
+// (0 = the default ZTile width). PP: lds holds two tiles (2 * NF * Z::LDS)
+// and the middle stages alternate between them, which drops the barrier
+// between a stage's reads and writes; with an odd number of middle stages the
+// last stage reads tile B, so the next transform's first-stage write to tile
+// A needs no barrier either. Between PP transforms a caller may not touch lds.
+template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, int A0, int B0, int A1, int B1>
 __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
                                      int tid) {
     using Z = ZTile<T, N, LINES>;
@@ -449,7 +468,10 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int i = 0; i < Z::T0; ++i) dft<R0, INV>(v[f][i]);
-        __syncthreads();  // previous readers of the tile buffers are done
+        // previous readers of tile A are done (ping-pong with an odd number of
+        // middle stages: the last reads of A were the middle stage's, before
+        // the barrier that ended it)
+        if constexpr (!PP || z_mid_stages<N>() % 2 == 0) __syncthreads();
         // unpadded rows: lane j writes element r ^ (j & SWM) in write r
         constexpr bool kPow2 = (R0 & (R0 - 1)) == 0;
         constexpr int SWM = S == NL && kPow2 ? (16 / NL < R0 ? 16 / NL : R0) - 1 : 0;
@@ -485,7 +507,7 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
             }
         __syncthreads();
         // stages 1 .. count-2 (NS after stage 0 is R0)
-        if constexpr (P.count > 2) zstages_mid<T, N, NL, NT, NF, INV, 1, R0>(lds, twl, tid);
+        const cx<T>* const last = zstages_mid<T, N, NL, NT, NF, INV, 1, R0, PP>(lds, lds + NF * Z::LDS, twl, tid);
         constexpr int NSL = N / RL;
 #pragma unroll
         for (int i = 0; i < Z::TLAST; ++i) {
@@ -493,7 +515,7 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
 #pragma unroll
-                for (int r = 0; r < RL; ++r) w[f][i][r] = lds[f * Z::LDS + (j + r * NSL) * S + l];
+                for (int r = 0; r < RL; ++r) w[f][i][r] = last[f * Z::LDS + (j + r * NSL) * S + l];
 #pragma unroll
                 for (int r = 1; r < RL; ++r) {
                     cx<T> tw = twl[r * j];

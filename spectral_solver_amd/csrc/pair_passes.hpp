@@ -101,10 +101,23 @@ __device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
     return {__shfl_xor(v.x, 1), __shfl_xor(v.y, 1)};
 }
 
+// Ping-pong z tiles (zfft's PP) in the solver's z passes: large lengths with
+// middle stages, when two field tiles and the twiddles fit in LDS (the
+// kernels already run one 1024-thread work-group per CU there).
+template <typename T, int N, int LINES = 0>
+constexpr bool z_pingpong() {
+    return N >= 256 && z_mid_stages<N>() >= 1 &&
+           (2 * ZTile<T, N, LINES>::LDS + N) * static_cast<int>(sizeof(cx<T>)) <= kLdsBytes;
+}
+template <typename T, int N, int LINES = 0>
+constexpr int z_lds() {
+    return (z_pingpong<T, N, LINES>() ? 2 : 1) * ZTile<T, N, LINES>::LDS;
+}
+
 // ---------------------------------------------------------------------------
 // inverse z: state -> pair fields. One tile: the body below runs in
 // k_pinv_z and, for the next RK stage, at the end of k_pfwd_z_rk_inv; the
-// twiddles are in twl already.
+// twiddles are in twl already, lds holds z_lds<T, N>() values.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int MODE>
 __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
@@ -165,7 +178,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
             for (int i = 0; i < T0; ++i)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) v[0][i][r] = f ? cmul(us[c][i][r], ph[i][r]) : us[c][i][r];
-            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
             store(a.out[c]);
         }
     } else if constexpr (MODE == PZ_DIAG) {
@@ -189,7 +202,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
                     const cx<T> uc = c == 0 ? u0 : (c == 1 ? u1 : u2);
                     v[0][i][r] = f ? om : uc;
                 }
-            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
             store(a.out[c]);
         }
     } else {  // PZ_VEL: pairs (u0, u1) and (u2, 0)
@@ -200,7 +213,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
 #pragma unroll
                 for (int r = 0; r < R0; ++r)
                     v[0][i][r] = q == 0 ? (f ? us[1][i][r] : us[0][i][r]) : (f ? czero<T>() : us[2][i][r]);
-            zfft<T, N, 1, true>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
             store(a.out[q]);
         }
     }
@@ -209,7 +222,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a) {
     using Z = ZTile<T, N>;
-    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> lds[z_lds<T, N>()];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
@@ -405,7 +418,13 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
     constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
-    __shared__ cx<T> lds[Z::LDS];
+    // ping-pong transforms; the pair separation then uses tile B, which the
+    // next transform writes only after its first barrier. Not at N = 1024,
+    // where the second tile's addressing pushes the 128-register x tile
+    // further into spills (measured 4% slower there, 3% faster at 512).
+    constexpr bool PP = N <= 512 && z_pingpong<T, N, LINES>();
+    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
+    cx<T>* const sep = PP ? lds + Z::LDS : lds;
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int l = tid % NL;
@@ -461,7 +480,7 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 }
                 v[0][i][r] = e <= N / 2 ? cx<T>{va.x - vb.y, va.y + vb.x} : cx<T>{va.x + vb.y, vb.x - va.y};
             }
-        zfft<T, N, 1, true, LINES>(v, w, lds, twl, tid);
+        zfft<T, N, 1, true, LINES, PP>(v, w, lds, twl, tid);
 #pragma unroll
         for (int i = 0; i < TLS; ++i)
 #pragma unroll
@@ -484,12 +503,12 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 const int m = i + r * T0;
                 v[0][i][r] = {u[PA[p]][m] * u[PB[p]][m], s[PA[p]][m] * s[PB[p]][m]};
             }
-        zfft<T, N, 1, false, LINES>(v, w, lds, twl, tid);
+        zfft<T, N, 1, false, LINES, PP>(v, w, lds, twl, tid);
         __syncthreads();  // the last stage's reads of the tile are done
 #pragma unroll
         for (int i = 0; i < TLS; ++i)
 #pragma unroll
-            for (int r = 0; r < RL; ++r) lds[Z::e_last(tid, i, r) * S + l] = w[0][i][r];
+            for (int r = 0; r < RL; ++r) sep[Z::e_last(tid, i, r) * S + l] = w[0][i][r];
         __syncthreads();
         cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw);
 #pragma unroll
@@ -498,8 +517,8 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
             if (ITEMS % NT == 0 || idx < ITEMS) {
                 const int l2 = idx % NL, k = idx / NL;
                 if (y0 + l2 < a.ny) {
-                    const cx<T> zk = lds[k * S + l2];
-                    const cx<T> zm = lds[((N - k) % N) * S + l2];
+                    const cx<T> zk = sep[k * S + l2];
+                    const cx<T> zm = sep[((N - k) % N) * S + l2];
                     cx2<T> q;
                     q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                     q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
@@ -617,7 +636,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 #pragma unroll
             for (int r = 0; r < R0; ++r) v[0][i][r] = vn[i][r];
         if (p < 5) load_pair(p + 1, vn);
-        zfft<T, N, 1, false>(v, w, lds, twl, tid);
+        zfft<T, N, 1, false, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
         // lane f = 0 holds P_hat, f = 1 holds Q_hat of every slot; swap halves
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
@@ -695,7 +714,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 template <typename T, int N>
 __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArgs<T> a, PairInvZArgs<T> inv) {
     using Z = ZTile<T, N>;
-    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> lds[z_lds<T, N>()];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const TileId tile = xcd_tile_id();
