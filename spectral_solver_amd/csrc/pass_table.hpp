@@ -9,7 +9,9 @@ namespace sph {
 
 // Variants of the tuned passes (inverse y, x, forward y) that the solver
 // times on its own buffers, keeping the fastest (Group::autotune):
-//   y passes: LDS-staged tiles of the default width, then half of it;
+//   y passes: LDS-staged tiles of the default width, half and a quarter of
+//             it (512^3 FP32 keeps the quarter: more, smaller work-groups per
+//             CU hide the latency of the tile loads better);
 //   x pass:   register-direct tiles of the default z-tile width, then half of
 //             it, then the LDS-staged form at the default width.
 constexpr int kTunedPasses = 3;
@@ -97,6 +99,7 @@ struct Launch {
     }
     static constexpr int L0 = TL::NL;
     static constexpr int L1 = L0 / 2 >= 2 ? L0 / 2 : L0;
+    static constexpr int L2 = L1 / 2 >= 2 ? L1 / 2 : L1;
     template <int L>
     static void plines_l(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                          const cx<T>* tw, hipStream_t s) {
@@ -109,7 +112,9 @@ struct Launch {
     }
     static void plines(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                        const cx<T>* tw, int variant, hipStream_t s) {
-        if (variant == 1)
+        if (variant == 2)
+            plines_l<L2>(data, npairlines, member_stride, members, inverse, tw, s);
+        else if (variant == 1)
             plines_l<L1>(data, npairlines, member_stride, members, inverse, tw, s);
         else
             plines_l<L0>(data, npairlines, member_stride, members, inverse, tw, s);
@@ -148,8 +153,8 @@ struct Launch {
     static constexpr PassTable<T> table{N,
                                         TL::NL,
                                         TL::NT,
-                                        {2, 3, 2},
-                                        {{L0, L1, 0}, {X0, X1, L0}, {L0, L1, 0}},
+                                        {3, 3, 3},
+                                        {{L0, L1, L2}, {X0, X1, L0}, {L0, L1, L2}},
                                         &inv_z,
                                         &lines,
                                         &inv_x,

@@ -277,7 +277,7 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch):
         monkeypatch.setenv("SPH_TILE_VARIANT", v)
         g = sph.TaylorGreenSolver(n, 400.0, "fp64")
         k = int(v)
-        assert [t[0] for t in g.tuning().values()] == [min(k, 1), k, min(k, 1)]
+        assert [t[0] for t in g.tuning().values()] == [k, k, k]
         for c in range(3):
             g.set_u_hat(c, oracle.c2c_to_half(u[c]))
         for _ in range(steps):
