@@ -104,6 +104,99 @@ def traffic_from_profiles(kernel: str, n: int):
     return float(m.group(1)) if m else None
 
 
+def fft_vs_rocfft(sph, n: int, reps: int = 10):
+    """One 3D real-to-complex forward and complex-to-real backward transform
+    of an n^3 FP32 field on device pointers: the library's FFT handle
+    (sph_fft_forward_device / backward_device, three pass kernels each)
+    against hipFFT/rocFFT plans of the same transforms (R2C, C2R), both on one
+    stream created here, timed with HIP events on that stream over `reps`
+    launches after two warm-ups. rocFFT is only timed here; nothing in the
+    library calls it."""
+    import ctypes
+    import numpy as np
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    fftlib = ctypes.CDLL("libhipfft.so.0")
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipStreamDestroy.argtypes = [vp]
+    hip.hipStreamSynchronize.argtypes = [vp]
+    fftlib.hipfftSetStream.argtypes = [vp, vp]
+    fftlib.hipfftPlan3d.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    fftlib.hipfftExecR2C.argtypes = [vp, vp, vp]
+    fftlib.hipfftExecC2R.argtypes = [vp, vp, vp]
+    fftlib.hipfftDestroy.argtypes = [vp]
+    HIPFFT_R2C, HIPFFT_C2R = 0x2A, 0x2C
+    rbytes = n ** 3 * 4
+    cbytes = n * n * (n // 2 + 1) * 8
+    bufs = [vp(), vp(), vp()]
+    for b, sz in zip(bufs, (rbytes, cbytes, rbytes)):
+        if hip.hipMalloc(ctypes.byref(b), sz) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ev = [vp(), vp()]
+    plans = [vp(), vp()]
+    st = vp()
+    try:
+        if hip.hipStreamCreate(ctypes.byref(st)) != 0:
+            raise RuntimeError("hipStreamCreate failed")
+        for e in ev:
+            hip.hipEventCreate(ctypes.byref(e))
+        u = np.random.default_rng(1).standard_normal(n ** 3).astype(np.float32)
+        hip.hipMemcpy(bufs[0], u.ctypes.data, rbytes, 1)
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            hip.hipEventRecord(ev[0], st)
+            for _ in range(reps):
+                fn()
+            hip.hipEventRecord(ev[1], st)
+            hip.hipEventSynchronize(ev[1])
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1])
+            return ms.value / reps
+
+        fft = sph.HipFft3D((n, n, n), (2 * 3.141592653589793,) * 3, precision="fp32")
+        ours_f = timed(lambda: fft.forward_device(bufs[0].value, bufs[1].value, st.value))
+        ours_b = timed(lambda: fft.backward_device(bufs[1].value, bufs[2].value, st.value))
+        hip.hipStreamSynchronize(st)
+        fft.close()
+        for q, kind in ((0, HIPFFT_R2C), (1, HIPFFT_C2R)):
+            if fftlib.hipfftPlan3d(ctypes.byref(plans[q]), n, n, n, kind) != 0 or \
+                    fftlib.hipfftSetStream(plans[q], st) != 0:
+                raise RuntimeError("hipFFT plan set-up failed")
+        roc_f = timed(lambda: fftlib.hipfftExecR2C(plans[0], bufs[0], bufs[1]))
+        roc_b = timed(lambda: fftlib.hipfftExecC2R(plans[1], bufs[1], bufs[2]))
+    finally:
+        if st.value:
+            hip.hipStreamSynchronize(st)
+        for p in plans:
+            if p.value:
+                fftlib.hipfftDestroy(p)
+        for e in ev:
+            if e.value:
+                hip.hipEventDestroy(e)
+        for b in bufs:
+            hip.hipFree(b)
+        if st.value:
+            hip.hipStreamDestroy(st)
+    gbs = lambda ms: round((rbytes + cbytes) / (ms * 1e-3) / 1e9, 1)
+    return {"workload": f"one 3D R2C / C2R transform of a {n}^3 FP32 field, device pointers", "reps": reps,
+            "ours_forward_ms": round(ours_f, 4), "rocfft_forward_ms": round(roc_f, 4),
+            "ours_backward_ms": round(ours_b, 4), "rocfft_backward_ms": round(roc_b, 4),
+            "min_traffic_gbs": {"ours_forward": gbs(ours_f), "rocfft_forward": gbs(roc_f),
+                                "ours_backward": gbs(ours_b), "rocfft_backward": gbs(roc_b)},
+            "note": "min_traffic_gbs = (one real + one half-spectrum field) / time; the backward includes "
+                    "the 1/N^3 scaling in ours, not in rocFFT's"}
+
+
 def cpu_baseline(n: int, dt: float):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -161,6 +254,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-fft-compare", action="store_true",
+                    help="skip the timed comparison of one 3D FFT with hipFFT/rocFFT")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--accuracy-to", type=float, default=None, metavar="T",
                     help="instead of timing: FP32 vs FP64 run to t=T, diagnostics every --accuracy-every")
@@ -246,6 +341,14 @@ def main():
         ens = ensemble_leg(sph, args.ensemble_steps, W)
         log("ensemble:", ens)
 
+    fftc = None
+    if not args.no_fft_compare:
+        try:
+            fftc = fft_vs_rocfft(sph, n)
+            log("fft vs rocfft:", fftc)
+        except Exception as e:  # a comparison only, never fatal for the GPU number
+            log("fft comparison failed:", e)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -277,6 +380,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "fft_vs_rocfft": fftc,
     }
     if fp64:
         out["energy_rel_err_fp32_vs_fp64"] = e_err
