@@ -264,18 +264,22 @@ def test_cli_fixed_dt(tmp_path):
     assert r.returncode == 1 and "error: grid size must be positive" in r.stderr
 
 
-def test_tile_variants_agree_and_match_oracle(monkeypatch):
+@pytest.mark.parametrize("precision,n", [("fp64", 32), ("fp32", 64)])
+def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
     """Every autotuner variant of the y and x passes (tile widths, and the
-    register-direct and LDS-staged x pass) gives the oracle's state at 1e-12
-    and the others' to rounding (the compiler contracts multiply-adds
-    differently in different kernels, so the bits may differ)."""
-    n, steps = 32, 3
+    register-direct and LDS-staged x pass) gives the oracle's state (FP64:
+    1e-12; FP32: the FP32 form's rounding) and the others' to rounding (the
+    compiler contracts multiply-adds differently in different kernels, so the
+    bits may differ). In FP32 at 64^3 the 8- and 4-line y tiles use the
+    unpadded rotated LDS rows and the 8-line x tile the unpadded z-tile rows
+    with the permuted first-stage write."""
+    steps = 3
     o, u = random_state(n, 11)
     states = []
     for v in ("0", "1", "2"):
-        print("case tile variant", v, flush=True)
+        print("case tile variant", precision, n, v, flush=True)
         monkeypatch.setenv("SPH_TILE_VARIANT", v)
-        g = sph.TaylorGreenSolver(n, 400.0, "fp64")
+        g = sph.TaylorGreenSolver(n, 400.0, precision)
         k = int(v)
         assert [t[0] for t in g.tuning().values()] == [k, k, k]
         for c in range(3):
@@ -286,10 +290,13 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch):
         g.close()
     for _ in range(steps):
         o.step_rk3(2e-3)
+    agree, match = (1e-14, 1e-12) if precision == "fp64" else (1e-6, 1e-6)  # FP32 measured 8e-8
     for st in states[1:]:
         num = sum(np.linalg.norm(st[c] - states[0][c]) ** 2 for c in range(3))
         den = sum(np.linalg.norm(states[0][c]) ** 2 for c in range(3))
-        assert np.sqrt(num / den) < 1e-14
-    num = sum(np.linalg.norm(states[0][c] - oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
-    den = sum(np.linalg.norm(oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
-    assert np.sqrt(num / den) < 1e-12
+        assert np.sqrt(num / den) < agree
+    for st in states:
+        num = sum(np.linalg.norm(st[c] - oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
+        den = sum(np.linalg.norm(oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
+        print("  rel L2 vs oracle", np.sqrt(num / den), flush=True)
+        assert np.sqrt(num / den) < match
