@@ -411,9 +411,10 @@ struct PairXArgs {
 // of Z_k and Z_{N-k}. Same formulas as the LDS-staged form below (results
 // agree to rounding; the compiler may contract multiply-adds differently).
 template <typename T, int N, int LINES = 0>
-// 512-thread tiles ask for two work-groups per CU (16 waves), which caps the
-// registers at 128 like the 1024-thread form.
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 2 : 1))
+// The second bound is the minimum number of waves per SIMD: 512-thread tiles
+// ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
+// at 128 like the 1024-thread form.
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
     k_px_products_rd(PairXArgs<T> a) {
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
