@@ -114,14 +114,29 @@ constexpr int z_lds() {
     return (z_pingpong<T, N, LINES>() ? 2 : 1) * ZTile<T, N, LINES>::LDS;
 }
 
+// State of a z tile kept in LDS between the update and the next stage's
+// inverse z (k_pfwd_z_rk_inv): component c, element (kz) e, ky ky_local of
+// the tile's NL/2; it overlays the transform tiles, so it needs
+// 3 * N * NL/2 values of the kernel's LDS.
+template <int N, int NL>
+__device__ __forceinline__ int z_stash_at(int c, int e, int kyl) {
+    return (c * N + e) * (NL / 2) + kyl;
+}
+template <typename T, int N>
+constexpr bool z_stash_fits() {
+    return 3 * N * (ZTile<T, N>::NL / 2) <= z_lds<T, N>();
+}
+
 // ---------------------------------------------------------------------------
 // inverse z: state -> pair fields. One tile: the body below runs in
 // k_pinv_z and, for the next RK stage, at the end of k_pfwd_z_rk_inv; the
 // twiddles are in twl already, lds holds z_lds<T, N>() values.
 // ---------------------------------------------------------------------------
+// su (optional): the tile's state in LDS, written by pfwd_z_rk_tile (layout
+// z_stash_at); the state is then not read from global memory.
 template <typename T, int N, int MODE>
 __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
-                                            int tid) {
+                                            int tid, const cx<T>* su = nullptr) {
     using Z = ZTile<T, N>;
     constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
@@ -156,12 +171,25 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
     // all three components of the lane's modes, read once (one 24/48-byte row
     // per mode of the component-interleaved state)
     cx<T> us[3][T0][R0];
+    if (su) {
+        const int kyl = static_cast<int>(kyc) - static_cast<int>(tile.x) * (NL / 2);
 #pragma unroll
-    for (int i = 0; i < T0; ++i)
+        for (int i = 0; i < T0; ++i)
 #pragma unroll
-        for (int r = 0; r < R0; ++r)
+            for (int r = 0; r < R0; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) us[c][i][r] = U.ld(uoff(i, r, c));
+                for (int c = 0; c < 3; ++c) us[c][i][r] = su[z_stash_at<N, NL>(c, Z::e_first(tid, i, r), kyl)];
+        // every wave has its values before the first transform writes over
+        // the stash
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int i = 0; i < T0; ++i)
+#pragma unroll
+            for (int r = 0; r < R0; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) us[c][i][r] = U.ld(uoff(i, r, c));
+    }
     if constexpr (MODE == PZ_RHS) {
         // member 0: u_hat_c, member 1: u_hat_c * phi (impl.hpp:866-873)
         cx<T> ph[T0][R0];
@@ -568,9 +596,11 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
 // forward z of the six product pairs + RHS completion + RK3 update (one tile;
 // the twiddles are in twl already)
 // ---------------------------------------------------------------------------
+// stash (optional): LDS for the updated state (z_stash_at), for the next
+// stage's inverse z in the same work-group.
 template <typename T, int N>
 __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
-                                               int tid) {
+                                               int tid, cx<T>* stash = nullptr) {
     using Z = ZTile<T, N>;
     constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     constexpr int SL = TLS * RL;  // last-stage slots per lane
@@ -661,6 +691,9 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
             r2[h] = r2[h] + ((pa == 2 ? da : czero<T>()) + (pb == 2 && pa != 2 ? db : czero<T>()));
         }
     }
+    // the stash overlays the transform tiles: all waves are past the last
+    // transform's reads before any lane writes it
+    if (stash) __syncthreads();
     if (!ok) return;
     const bool first = a.stage == 0;
     const Buf<T> U(u3 + plane * 3, N * ny * 3 * CB);
@@ -702,6 +735,12 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
             G.st(off[h] + c * CB, g[h][c]);
             U.st(off[h] + c * CB, u[h][c]);
         }
+    if (stash) {
+#pragma unroll
+        for (int h = 0; h < HS; ++h)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) stash[z_stash_at<N, NL>(c, owned_e(h), l >> 1)] = u[h][c];
+    }
 }
 
 // The last pass of an RK stage: forward z + RHS completion + update of the
@@ -720,13 +759,17 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArg
     const int tid = threadIdx.x;
     const TileId tile = xcd_tile_id();
     load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
-    pfwd_z_rk_tile<T, N>(a, tile, lds, twl, tid);
-    // work-group scope is enough for the read-back: the barrier's release /
-    // acquire fences make every wave's state stores visible to the others
-    // (one CU, one L1). A device-scope fence here would write back the XCD's
-    // whole L2 per work-group (measured: 6x slower pass).
+    // the updated state goes to the next stage's inverse z through LDS when
+    // it fits beside nothing (it overlays the transform tiles), otherwise it
+    // is read back from global memory (L2). Work-group scope is enough for
+    // that read-back: the barrier's release / acquire fences make every
+    // wave's state stores visible to the others (one CU, one L1); a
+    // device-scope fence would write back the XCD's whole L2 per work-group
+    // (measured: 6x slower pass).
+    constexpr bool kStash = z_stash_fits<T, N>();
+    pfwd_z_rk_tile<T, N>(a, tile, lds, twl, tid, kStash ? lds : nullptr);
     __syncthreads();
-    pinv_z_tile<T, N, PZ_RHS>(inv, tile, lds, twl, tid);
+    pinv_z_tile<T, N, PZ_RHS>(inv, tile, lds, twl, tid, kStash ? lds : nullptr);
 }
 
 // max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)
