@@ -747,8 +747,8 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 // tile's (kx, ky) lines, then the next stage's inverse z of the same lines
 // (impl.hpp:979-1045 runs compute_rhs's backward transforms of u_hat at the
 // start of every stage; here they follow the update that produced u_hat, so
-// the state is read back from L2 rather than from HBM and one launch per
-// stage goes away). The inverse half writes the velocity pairs into
+// the state is not read from HBM again and one launch per stage goes away).
+// The inverse half writes the velocity pairs into
 // inv.out[0..2] (the first three product pair fields, whose values the
 // forward half has consumed: every work-group reads its own lines only).
 template <typename T, int N>
@@ -760,12 +760,12 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArg
     const TileId tile = xcd_tile_id();
     load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
     // the updated state goes to the next stage's inverse z through LDS when
-    // it fits beside nothing (it overlays the transform tiles), otherwise it
-    // is read back from global memory (L2). Work-group scope is enough for
-    // that read-back: the barrier's release / acquire fences make every
-    // wave's state stores visible to the others (one CU, one L1); a
-    // device-scope fence would write back the XCD's whole L2 per work-group
-    // (measured: 6x slower pass).
+    // 3 * N * NL/2 values fit in the kernel's LDS (the stash overlays the
+    // transform tiles), otherwise it is read back from global memory (L2).
+    // Work-group scope is enough for that read-back: the barrier's release /
+    // acquire fences make every wave's state stores visible to the others
+    // (one CU, one L1); a device-scope fence would write back the XCD's
+    // whole L2 per work-group (measured: 6x slower pass).
     constexpr bool kStash = z_stash_fits<T, N>();
     pfwd_z_rk_tile<T, N>(a, tile, lds, twl, tid, kStash ? lds : nullptr);
     __syncthreads();
