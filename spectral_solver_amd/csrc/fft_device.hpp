@@ -325,10 +325,11 @@ __device__ __forceinline__ void fft_tile(cx<T>* buf, const cx<T>* twl, int tid) 
 // once: stored unpadded, every access whose 16 lanes touch 16/NL consecutive
 // rows is conflict-free (the middle and last stages, and the x pass's own
 // accesses). The first-stage write is the exception: its lanes write rows
-// j*R0 + r for consecutive j, R0 apart, which share banks; there lane j writes
-// its values in the order r ^ (j mod 16/NL), so that a write instruction's
-// rows differ by R0 +- a few and land in distinct banks. Wider FP32 rows and
-// FP64 rows keep one element of padding per row.
+// j*R0 + r for the butterflies j of neighbouring lane groups, a multiple of R0
+// apart, which share banks; there lane group p = tid/NL writes its values in
+// the order r ^ (p mod 16/NL), so that a write instruction's rows land in
+// distinct banks (also with the mirrored butterfly order, j_mirror). Wider
+// FP32 rows and FP64 rows keep one element of padding per row.
 // ---------------------------------------------------------------------------
 template <typename T>
 constexpr int ztile_stride(int nl) {
@@ -372,6 +373,21 @@ struct ZTile {
     // element index of first-stage slot (i, r) / last-stage slot (i, r)
     __device__ static int e_first(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / R0); }
     __device__ static int e_last(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / RL); }
+    // First-stage butterflies in mirrored order (zfft's MIRROR): lane group
+    // p = tid / NL takes butterfly j = 0, STEP/2, 1, STEP-1, 2, STEP-2, ...
+    // for p = 0, 1, 2, 3, ..., so that the groups of j and STEP - j are
+    // neighbours in one wavefront. With one task per thread and N/R0 = STEP,
+    // element e = j + r*STEP of group j and element N - e of group STEP - j
+    // (slot R0-1-r) are then loaded by the same wavefront: the x pass's
+    // Hermitian extension reads each spectral row twice, and the second read
+    // now comes from the same wavefront's recent access.
+    static constexpr bool kMirrorable = T0 == 1 && N / R0 == STEP && STEP % 2 == 0 && 64 / NL >= 2;
+    __device__ static int j_mirror(int tid) {
+        const int p = tid / NL;
+        const int m = p >> 1;
+        return p < 2 ? p * (STEP / 2) : ((p & 1) ? STEP - m : m);
+    }
+    __device__ static int e_first_mirror(int tid, int r) { return j_mirror(tid) + r * (N / R0); }
 };
 
 // One middle Stockham stage over NF field tiles. In place (src == dst): read
@@ -446,11 +462,13 @@ constexpr int kLdsBytes = 160 * 1024;  // LDS per work-group on gfx950
 // between a stage's reads and writes; with an odd number of middle stages the
 // last stage reads tile B, so the next transform's first-stage write to tile
 // A needs no barrier either. Between PP transforms a caller may not touch lds.
-template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, int A0, int B0, int A1, int B1>
+template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false, int A0, int B0,
+          int A1, int B1>
 __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
                                      int tid) {
     using Z = ZTile<T, N, LINES>;
     static_assert(A0 == Z::T0 && B0 == Z::R0 && A1 == Z::TLAST && B1 == Z::RL, "slot arrays must match the tile");
+    static_assert(!MIRROR || Z::kMirrorable, "mirrored first stage needs one task per thread");
     constexpr int NL = Z::NL, NT = Z::NT, S = Z::S, R0 = Z::R0, RL = Z::RL;
     constexpr RadixPlan P = Z::P;
     if constexpr (P.count == 1) {
@@ -472,20 +490,21 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
         // middle stages: the last reads of A were the middle stage's, before
         // the barrier that ended it)
         if constexpr (!PP || z_mid_stages<N>() % 2 == 0) __syncthreads();
-        // unpadded rows: lane j writes element r ^ (j & SWM) in write r
+        // unpadded rows: lane group p = tid / NL writes element r ^ (p & SWM) in
+        // write r
         constexpr bool kPow2 = (R0 & (R0 - 1)) == 0;
         constexpr int SWM = S == NL && kPow2 ? (16 / NL < R0 ? 16 / NL : R0) - 1 : 0;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int i = 0; i < Z::T0; ++i) {
-                const int j = tid / NL + i * Z::STEP;
+                const int j = MIRROR ? Z::j_mirror(tid) : tid / NL + i * Z::STEP;
                 cx<T>* const row0 = lds + f * Z::LDS + j * R0 * S + l;
                 if constexpr (SWM == 0) {
 #pragma unroll
                     for (int r = 0; r < R0; ++r) row0[r * S] = v[f][i][r];
                 } else {
-                    const int sw = j & SWM;
+                    const int sw = (tid / NL) & SWM;  // lane group parity (also in mirrored order)
                     cx<T> u[R0];
 #pragma unroll
                     for (int r = 0; r < R0; ++r) u[r] = v[f][i][r];
