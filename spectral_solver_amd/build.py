@@ -19,6 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libspectral_hip.so")
 CLI = os.path.join(HERE, "taylor_green")
 CPP_TESTS = os.path.join(HERE, "spectral_ns_tests")
+INDEX_TESTS = os.path.join(HERE, "index_tests")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -60,7 +61,9 @@ def build(verbose: bool = True) -> str:
     # host programs over the C ABI: the CLI and the reference-style C++ test
     host = [HIPCC, "-O2", "-std=c++17", "-I" + INCLUDE, "-Wl,-rpath,$ORIGIN", "-L" + HERE, "-lspectral_hip"]
     _run([[*host[:4], os.path.join(CSRC, "main_taylor_green.cpp"), "-o", CLI, *host[4:]],
-          [*host[:4], os.path.join(CSRC, "spectral_ns_tests.cpp"), "-o", CPP_TESTS, *host[4:]]])
+          [*host[:4], os.path.join(CSRC, "spectral_ns_tests.cpp"), "-o", CPP_TESTS, *host[4:]],
+          # host checks of the kernels' index maps (no GPU needed to run)
+          [HIPCC, *HIP_FLAGS, os.path.join(CSRC, "index_tests.hip"), "-o", INDEX_TESTS]])
     if verbose:
         print("[build] wrote", LIB, flush=True)
     return LIB

@@ -216,7 +216,7 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 // order r ^ (j mod 16/NL) (fft_stage).
 // ---------------------------------------------------------------------------
 template <int NL, bool SWZ>
-__device__ __forceinline__ int tile_at(int row, int line) {
+__host__ __device__ __forceinline__ int tile_at(int row, int line) {
     if constexpr (SWZ) {
         static_assert(NL == 2 || NL == 4 || NL == 8, "swizzled rows hold 2, 4 or 8 lines");
         constexpr unsigned RB = 16 / NL;
@@ -371,8 +371,8 @@ struct ZTile {
     static_assert(NT <= 1024 && NT % 64 == 0 && NT % NL == 0, "bad z tile");
     static_assert(E % R0 == 0 && E % RL == 0, "bad z tile");
     // element index of first-stage slot (i, r) / last-stage slot (i, r)
-    __device__ static int e_first(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / R0); }
-    __device__ static int e_last(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / RL); }
+    __host__ __device__ static int e_first(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / R0); }
+    __host__ __device__ static int e_last(int tid, int i, int r) { return tid / NL + i * STEP + r * (N / RL); }
     // First-stage butterflies in mirrored order (zfft's MIRROR): lane group
     // p = tid / NL takes butterfly j = 0, STEP/2, 1, STEP-1, 2, STEP-2, ...
     // for p = 0, 1, 2, 3, ..., so that the groups of j and STEP - j are
@@ -382,12 +382,12 @@ struct ZTile {
     // Hermitian extension reads each spectral row twice, and the second read
     // now comes from the same wavefront's recent access.
     static constexpr bool kMirrorable = T0 == 1 && N / R0 == STEP && STEP % 2 == 0 && 64 / NL >= 2;
-    __device__ static int j_mirror(int tid) {
+    __host__ __device__ static int j_mirror(int tid) {
         const int p = tid / NL;
         const int m = p >> 1;
         return p < 2 ? p * (STEP / 2) : ((p & 1) ? STEP - m : m);
     }
-    __device__ static int e_first_mirror(int tid, int r) { return j_mirror(tid) + r * (N / R0); }
+    __host__ __device__ static int e_first_mirror(int tid, int r) { return j_mirror(tid) + r * (N / R0); }
 };
 
 // One middle Stockham stage over NF field tiles. In place (src == dst): read

@@ -1,0 +1,119 @@
+// Host checks of the index maps the pass kernels rely on: every LDS layout
+// and every permutation of butterflies or slots must be a bijection, or a
+// kernel would silently overwrite or skip values. Built by build.py, run by
+// tests/test_index_maps.py (no GPU needed).
+#include <cstdio>
+#include <vector>
+
+#include "pass_table.hpp"
+
+namespace {
+
+int failures = 0;
+
+void expect(bool ok, const char* what, int a, int b) {
+    if (!ok) {
+        ++failures;
+        std::printf("FAIL %s (%d, %d)\n", what, a, b);
+    }
+}
+
+// tile_at<NL, true>: (row, line) -> slot is a bijection onto [0, N*NL)
+template <int NL>
+void swizzled_tile(int n) {
+    std::vector<int> seen(static_cast<std::size_t>(n) * NL, 0);
+    for (int row = 0; row < n; ++row)
+        for (int line = 0; line < NL; ++line) {
+            const int a = sph::tile_at<NL, true>(row, line);
+            expect(a >= 0 && a < n * NL, "tile_at range", row, line);
+            if (a >= 0 && a < n * NL) ++seen[a];
+        }
+    for (int a = 0; a < n * NL; ++a) expect(seen[a] == 1, "tile_at bijection", n, a);
+}
+
+// z_stash_at<N, NL>: (component, element, ky) -> slot is a bijection onto
+// [0, 3*N*NL/2) and fits in the kernel's LDS when the kernel uses it
+template <typename T, int N>
+void stash() {
+    constexpr int NL = sph::ZTile<T, N>::NL;
+    std::vector<int> seen(3 * N * (NL / 2), 0);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < N; ++e)
+            for (int k = 0; k < NL / 2; ++k) {
+                const int a = sph::z_stash_at<N, NL>(c, e, k);
+                expect(a >= 0 && a < 3 * N * (NL / 2), "stash range", e, k);
+                if (a >= 0 && a < 3 * N * (NL / 2)) ++seen[a];
+            }
+    for (int a = 0; a < 3 * N * (NL / 2); ++a) expect(seen[a] == 1, "stash bijection", N, a);
+    if (sph::z_stash_fits<T, N>()) expect(3 * N * (NL / 2) <= sph::z_lds<T, N>(), "stash fits", N, NL);
+    expect(sph::z_lds<T, N>() * static_cast<int>(sizeof(sph::cx<T>)) + N * static_cast<int>(sizeof(sph::cx<T>)) <=
+               sph::kLdsBytes,
+           "z tile LDS", N, NL);
+}
+
+// ZTile::j_mirror: a bijection of the first-stage butterflies in which the
+// groups of j and STEP - j are neighbours (p, p + 1 with p even) in one
+// wavefront, and element e of one is element N - e of the other
+template <typename T, int N, int LINES>
+void mirror() {
+    using Z = sph::ZTile<T, N, LINES>;
+    if constexpr (Z::kMirrorable) {
+        std::vector<int> seen(Z::STEP, 0);
+        for (int p = 0; p < Z::STEP; ++p) {
+            const int j = Z::j_mirror(p * Z::NL);
+            expect(j >= 0 && j < Z::STEP, "mirror range", N, p);
+            if (j >= 0 && j < Z::STEP) ++seen[j];
+            if (p >= 2 && p % 2 == 0) {
+                const int jn = Z::j_mirror((p + 1) * Z::NL);
+                expect(j + jn == Z::STEP, "mirror partner", N, p);
+                expect((p * Z::NL) / 64 == ((p + 1) * Z::NL) / 64, "mirror same wavefront", N, p);
+                for (int r = 0; r < Z::R0; ++r)
+                    expect(Z::e_first_mirror(p * Z::NL, r) + Z::e_first_mirror((p + 1) * Z::NL, Z::R0 - 1 - r) == N,
+                           "mirror elements", N, r);
+            }
+        }
+        for (int j = 0; j < Z::STEP; ++j) expect(seen[j] == 1, "mirror bijection", N, j);
+    }
+}
+
+// the first-stage write order r ^ s for every lane-group value s <= SWM is a
+// permutation of the R0 slots (zfft, fft_stage)
+void xor_orders() {
+    for (int r0 : {2, 4, 8})
+        for (int s = 0; s < r0; ++s) {
+            std::vector<int> seen(r0, 0);
+            for (int r = 0; r < r0; ++r) ++seen[r ^ s];
+            for (int r = 0; r < r0; ++r) expect(seen[r] == 1, "xor order", r0, s);
+        }
+}
+
+}  // namespace
+
+int main() {
+    for (int n : {8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024}) {
+        swizzled_tile<2>(n);
+        swizzled_tile<4>(n);
+        swizzled_tile<8>(n);
+    }
+    stash<float, 256>();
+    stash<float, 512>();
+    stash<float, 1024>();
+    stash<double, 256>();
+    stash<double, 512>();
+    stash<double, 1024>();
+    mirror<float, 32, 0>();
+    mirror<float, 64, 0>();
+    mirror<float, 256, 0>();
+    mirror<float, 512, 0>();
+    mirror<float, 512, 8>();
+    mirror<float, 1024, 0>();
+    mirror<double, 512, 0>();
+    mirror<double, 256, 0>();
+    xor_orders();
+    if (failures) {
+        std::printf("%d index-map checks failed\n", failures);
+        return 1;
+    }
+    std::printf("all index-map checks passed\n");
+    return 0;
+}

@@ -119,7 +119,7 @@ constexpr int z_lds() {
 // the tile's NL/2; it overlays the transform tiles, so it needs
 // 3 * N * NL/2 values of the kernel's LDS.
 template <int N, int NL>
-__device__ __forceinline__ int z_stash_at(int c, int e, int kyl) {
+__host__ __device__ __forceinline__ int z_stash_at(int c, int e, int kyl) {
     return (c * N + e) * (NL / 2) + kyl;
 }
 template <typename T, int N>
