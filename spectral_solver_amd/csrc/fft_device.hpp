@@ -84,24 +84,35 @@ constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 
 
 // Plan of the register-direct z passes, where every thread holds lcm(radices)
 // elements per field: a radix-8 stage next to a radix-3 one would make that 24,
-// so there each 8 is split into 4*2 (12 elements per thread).
+// so lengths with a factor 3 take radix-4 stages and, for an odd power of two,
+// one radix-6 stage (2*3) in place of a 2 and a 3: 12 elements per thread and
+// the fewest stages (96 = 4*4*6, 24 = 4*6, 48 = 4*4*3, 12 = 4*3).
 constexpr RadixPlan z_radix_plan(int n) {
     const RadixPlan p = radix_plan(n);
-    bool has8 = false, has3 = false;
-    for (int s = 0; s < p.count; ++s) {
-        has8 |= p.r[s] == 8;
-        has3 |= p.r[s] == 3;
+    bool has3 = false;
+    for (int s = 0; s < p.count; ++s) has3 |= p.r[s] == 3;
+    if (!has3 || p.count < 0) return p;
+    int twos = 0, threes = 0, fives = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        ++twos;
     }
-    if (!(has8 && has3)) return p;
+    while (n % 3 == 0) {
+        n /= 3;
+        ++threes;
+    }
+    while (n % 5 == 0) {
+        n /= 5;
+        ++fives;
+    }
     RadixPlan q{};
-    for (int s = 0; s < p.count; ++s) {
-        if (p.r[s] == 8) {
-            q.r[q.count++] = 4;
-            q.r[q.count++] = 2;
-        } else {
-            q.r[q.count++] = p.r[s];
-        }
+    for (int i = 0; i < twos / 2; ++i) q.r[q.count++] = 4;
+    if (twos % 2) {
+        q.r[q.count++] = 6;
+        --threes;
     }
+    for (int i = 0; i < threes; ++i) q.r[q.count++] = 3;
+    for (int i = 0; i < fives; ++i) q.r[q.count++] = 5;
     return q;
 }
 
@@ -181,6 +192,25 @@ __device__ __forceinline__ void dft(cx<T>* v) {
         v[0] = v[0] + t;
         v[1] = m + d;
         v[2] = m - d;
+    } else if constexpr (R == 6) {
+        // 6 = 2 * 3: radix-3 DFTs of the even and odd elements, then radix-2
+        // combinations with the twiddles w6^k, w6 = e^{-+2 pi i/6}
+        const T c = T(0.5);
+        const T sn = T(0.866025403784438646763723170752936183L);  // sin(pi/3)
+        cx<T> a3[3] = {v[0], v[2], v[4]};
+        cx<T> b3[3] = {v[1], v[3], v[5]};
+        dft<3, INV>(a3);
+        dft<3, INV>(b3);
+        // w6 = (1/2, -+sin(pi/3)), w6^2 = (-1/2, -+sin(pi/3))
+        const cx<T> w1 = INV ? cx<T>{c, sn} : cx<T>{c, -sn};
+        const cx<T> w2 = INV ? cx<T>{-c, sn} : cx<T>{-c, -sn};
+        const cx<T> t1 = cmul(b3[1], w1), t2 = cmul(b3[2], w2);
+        v[0] = a3[0] + b3[0];
+        v[3] = a3[0] - b3[0];
+        v[1] = a3[1] + t1;
+        v[4] = a3[1] - t1;
+        v[2] = a3[2] + t2;
+        v[5] = a3[2] - t2;
     } else if constexpr (R == 5) {
         // y_k = sum_q x_q w^{qk}, w = e^{-+2 pi i/5}
         const T c1 = T(0.309016994374947424102293417182819059L);   // cos(2pi/5)
