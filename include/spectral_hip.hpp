@@ -395,6 +395,39 @@ inline void write_diagnostics_csv(const std::string& path, const std::vector<Dia
           << '\n';
 }
 
+// Binary PPM (P6) image of |ω| on the mid z plane iz = n_z/2, scaled by the
+// plane's maximum (1 when the plane is zero), rows from iy = n_y-1 down to 0;
+// colour of q = |ω|/max: r = 255·√q, g = 255·q², b = 255·(1−q)/4, each
+// truncated to a byte (src/spectral_solver.cpp:421-486). One rank holds the
+// whole plane, so nothing is gathered.
+inline void write_vorticity_ppm(const std::string& path, SpectralNavierStokes& solver, const VectorSpectral& u_hat) {
+    const auto w = solver.to_physical(solver.vorticity_hat(u_hat));
+    const auto& n = solver.fft().shape();
+    const int iz = n[2] / 2;
+    const std::size_t plane = static_cast<std::size_t>(n[0]) * n[1];
+    std::vector<double> mag(plane);
+    double peak = 0.0;
+    for (std::size_t i = 0; i < plane; ++i) {
+        const std::size_t id = static_cast<std::size_t>(iz) * plane + i;   // x fastest, then y, then z
+        mag[i] = std::sqrt(w[0][id] * w[0][id] + w[1][id] * w[1][id] + w[2][id] * w[2][id]);
+        peak = std::max(peak, mag[i]);
+    }
+    if (peak == 0.0) peak = 1.0;
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    f << "P6\n" << n[0] << " " << n[1] << "\n255\n";
+    std::vector<unsigned char> row(3 * static_cast<std::size_t>(n[0]));
+    for (int iy = n[1] - 1; iy >= 0; --iy) {
+        for (int ix = 0; ix < n[0]; ++ix) {
+            const double q = std::clamp(mag[static_cast<std::size_t>(iy) * n[0] + ix] / peak, 0.0, 1.0);
+            row[3 * ix + 0] = static_cast<unsigned char>(255.0 * std::sqrt(q));
+            row[3 * ix + 1] = static_cast<unsigned char>(255.0 * q * q);
+            row[3 * ix + 2] = static_cast<unsigned char>(255.0 * (1.0 - q) * 0.25);
+        }
+        f.write(reinterpret_cast<const char*>(row.data()), static_cast<std::streamsize>(row.size()));
+    }
+}
+
 namespace ava_solver {
 
 struct Diagnostics {

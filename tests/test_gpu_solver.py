@@ -244,9 +244,9 @@ def test_time_passes_leaves_state_unchanged():
         assert np.array_equal(before[c], s.u_hat(c))
 
 
-def test_reference_cpp_tests_binary():
+def test_reference_cpp_tests_binary(tmp_path):
     exe = os.path.join(ROOT, "spectral_solver_amd", "spectral_ns_tests")
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, str(tmp_path / "vorticity.ppm")], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0 and "all C++ spectral tests passed" in r.stdout
 
