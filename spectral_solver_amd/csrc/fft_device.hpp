@@ -418,6 +418,13 @@ struct ZTile {
         return p < 2 ? p * (STEP / 2) : ((p & 1) ? STEP - m : m);
     }
     __host__ __device__ static int e_first_mirror(int tid, int r) { return j_mirror(tid) + r * (N / R0); }
+    // Last-stage butterflies in the same order (zfft's MIRROR_LAST): with one
+    // task per thread and N/RL = STEP, output element e = j + r*STEP of lane
+    // group p >= 2 and element N - e (slot RL-1-r) of group p ^ 1 are one
+    // lane exchange apart (lane ^ NL); groups 0 and 1 hold j = 0 and STEP/2,
+    // whose elements N - e are their own slots (RL - r) % RL and RL-1-r.
+    static constexpr bool kMirrorableLast = TLAST == 1 && N / RL == STEP && STEP % 2 == 0 && 64 / NL >= 2;
+    __host__ __device__ static int e_last_mirror(int tid, int r) { return j_mirror(tid) + r * (N / RL); }
 };
 
 // One middle Stockham stage over NF field tiles. In place (src == dst): read
@@ -492,13 +499,16 @@ constexpr int kLdsBytes = 160 * 1024;  // LDS per work-group on gfx950
 // between a stage's reads and writes; with an odd number of middle stages the
 // last stage reads tile B, so the next transform's first-stage write to tile
 // A needs no barrier either. Between PP transforms a caller may not touch lds.
-template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false, int A0, int B0,
-          int A1, int B1>
+// MIRROR / MIRROR_LAST: first / last-stage butterflies in ZTile::j_mirror
+// order (w then holds the elements e_last_mirror).
+template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false,
+          bool MIRROR_LAST = false, int A0, int B0, int A1, int B1>
 __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
                                      int tid) {
     using Z = ZTile<T, N, LINES>;
     static_assert(A0 == Z::T0 && B0 == Z::R0 && A1 == Z::TLAST && B1 == Z::RL, "slot arrays must match the tile");
     static_assert(!MIRROR || Z::kMirrorable, "mirrored first stage needs one task per thread");
+    static_assert(!MIRROR_LAST || Z::kMirrorableLast, "mirrored last stage needs one task per thread");
     constexpr int NL = Z::NL, NT = Z::NT, S = Z::S, R0 = Z::R0, RL = Z::RL;
     constexpr RadixPlan P = Z::P;
     if constexpr (P.count == 1) {
@@ -560,7 +570,7 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
         constexpr int NSL = N / RL;
 #pragma unroll
         for (int i = 0; i < Z::TLAST; ++i) {
-            const int j = tid / NL + i * Z::STEP;
+            const int j = MIRROR_LAST ? Z::j_mirror(tid) : tid / NL + i * Z::STEP;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
 #pragma unroll

@@ -76,6 +76,40 @@ void mirror() {
     }
 }
 
+// ZTile::e_last_mirror: for lane group p, the partner of output slot r
+// (element N - e) is slot RL-1-r of group p ^ 1 (p >= 2), own slot
+// (RL - r) % RL (p = 0) or own slot RL-1-r (p = 1); partners share a
+// wavefront; the rows k = j + r*STEP, r < RL/2, plus N/2 from group 0 cover
+// 0 .. N/2 once (the x pass's lane-exchange pair separation)
+template <typename T, int N, int LINES>
+void mirror_last() {
+    using Z = sph::ZTile<T, N, LINES>;
+    if constexpr (Z::kMirrorableLast) {
+        constexpr int RL = Z::RL, NL = Z::NL;
+        std::vector<int> rows(N / 2 + 1, 0);
+        for (int p = 0; p < Z::STEP; ++p) {
+            const int tid = p * NL;
+            for (int r = 0; r < RL; ++r) {
+                const int e = Z::e_last_mirror(tid, r);
+                int partner;
+                if (p == 0)
+                    partner = Z::e_last_mirror(tid, (RL - r) % RL);
+                else if (p == 1)
+                    partner = Z::e_last_mirror(tid, RL - 1 - r);
+                else
+                    partner = Z::e_last_mirror((p ^ 1) * NL, RL - 1 - r);
+                expect((e + partner) % N == 0, "last mirror partner", N, p * RL + r);
+                if (p >= 2) expect(tid / 64 == ((p ^ 1) * NL) / 64, "last mirror same wavefront", N, p);
+                if (r < RL / 2 || (p == 0 && r == RL / 2)) {
+                    expect(e >= 0 && e <= N / 2, "last mirror row range", N, e);
+                    if (e >= 0 && e <= N / 2) ++rows[e];
+                }
+            }
+        }
+        for (int k = 0; k <= N / 2; ++k) expect(rows[k] == 1, "last mirror rows", N, k);
+    }
+}
+
 // the first-stage write order r ^ s for every lane-group value s <= SWM is a
 // permutation of the R0 slots (zfft, fft_stage)
 void xor_orders() {
@@ -109,6 +143,16 @@ int main() {
     mirror<float, 1024, 0>();
     mirror<double, 512, 0>();
     mirror<double, 256, 0>();
+    mirror_last<float, 32, 0>();
+    mirror_last<float, 64, 0>();
+    mirror_last<float, 128, 0>();
+    mirror_last<float, 256, 0>();
+    mirror_last<float, 512, 0>();
+    mirror_last<float, 512, 8>();
+    mirror_last<double, 64, 0>();
+    mirror_last<double, 128, 0>();
+    mirror_last<double, 256, 0>();
+    mirror_last<double, 512, 0>();
     xor_orders();
     if (failures) {
         std::printf("%d index-map checks failed\n", failures);

@@ -459,6 +459,11 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     // wavefront (ZTile::j_mirror); measured 2% faster at 512, 3% slower at
     // 1024 (8-line tiles)
     constexpr bool MIR = N <= 512 && Z::kMirrorable;
+    // forward transforms: last-stage butterflies in the same mirrored order,
+    // so Z_k and Z_{N-k} of a line sit in lanes tid and tid ^ NL of one
+    // wavefront and the pair separation is a lane exchange instead of a trip
+    // through an LDS tile with two barriers
+    constexpr bool MIRL = N <= 512 && Z::kMirrorableLast;
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int l = tid % NL;
@@ -537,7 +542,34 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 const int m = i + r * T0;
                 v[0][i][r] = {u[PA[p]][m] * u[PB[p]][m], s[PA[p]][m] * s[PB[p]][m]};
             }
-        zfft<T, N, 1, false, LINES, PP>(v, w, lds, twl, tid);
+        zfft<T, N, 1, false, LINES, PP, false, MIRL>(v, w, lds, twl, tid);
+        if constexpr (MIRL) {
+            // element j + r*STEP for r < RL/2 (and N/2 in group j = 0): rows
+            // k = 0 .. N/2 once over the work-group
+            constexpr int HR = RL / 2;
+            const int j = Z::j_mirror(tid);
+            const bool self0 = j == 0, selfh = j == Z::STEP / 2;
+            cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw) + static_cast<size_t>(z) * a.ny + y0 + l;
+#pragma unroll
+            for (int r = 0; r <= HR; ++r) {
+                const cx<T> zk = w[0][0][r % RL];
+                cx<T> zm;
+                if (r < HR) {
+                    const cx<T> mine = w[0][0][RL - 1 - r];
+                    cx<T> other;
+                    other.x = __shfl_xor(mine.x, NL);
+                    other.y = __shfl_xor(mine.y, NL);
+                    zm = self0 ? w[0][0][(RL - r) % RL] : (selfh ? mine : other);
+                } else {
+                    zm = zk;  // k = N/2, group j = 0 only
+                }
+                cx2<T> q;
+                q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+                q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+                if (ok && (r < HR || self0)) o[static_cast<size_t>(j + r * Z::STEP) * row] = q;
+            }
+            continue;
+        }
         __syncthreads();  // the last stage's reads of the tile are done
 #pragma unroll
         for (int i = 0; i < TLS; ++i)
