@@ -86,12 +86,27 @@ constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 
 // elements per field: a radix-8 stage next to a radix-3 one would make that 24,
 // so lengths with a factor 3 take radix-4 stages and, for an odd power of two,
 // one radix-6 stage (2*3) in place of a 2 and a 3: 12 elements per thread and
-// the fewest stages (96 = 4*4*6, 24 = 4*6, 48 = 4*4*3, 12 = 4*3).
+// the fewest stages (96 = 4*4*6, 24 = 4*6, 48 = 4*4*3, 12 = 4*3). Powers of
+// two with two or more radix-8 stages put one of them last (1024 = 8*4*4*8,
+// 256 = 8*4*8): the same stages, and the last one then holds one task per
+// thread (ZTile::kMirrorableLast).
 constexpr RadixPlan z_radix_plan(int n) {
     const RadixPlan p = radix_plan(n);
+    if (p.count < 0) return p;
     bool has3 = false;
-    for (int s = 0; s < p.count; ++s) has3 |= p.r[s] == 3;
-    if (!has3 || p.count < 0) return p;
+    int eights = 0;
+    for (int s = 0; s < p.count; ++s) {
+        has3 |= p.r[s] == 3;
+        eights += p.r[s] == 8;
+    }
+    if (!has3) {
+        RadixPlan q = p;
+        if (eights >= 2 && p.r[0] == 8 && p.r[1] == 8 && p.r[p.count - 1] != 8) {
+            for (int s = 1; s < p.count - 1; ++s) q.r[s] = p.r[s + 1];
+            q.r[p.count - 1] = 8;
+        }
+        return q;
+    }
     int twos = 0, threes = 0, fives = 0;
     while (n % 2 == 0) {
         n /= 2;

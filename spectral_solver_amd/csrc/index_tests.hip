@@ -149,6 +149,9 @@ int main() {
     mirror_last<float, 256, 0>();
     mirror_last<float, 512, 0>();
     mirror_last<float, 512, 8>();
+    mirror_last<float, 1024, 0>();
+    mirror_last<float, 1024, 4>();
+    mirror_last<double, 1024, 0>();
     mirror_last<double, 64, 0>();
     mirror_last<double, 128, 0>();
     mirror_last<double, 256, 0>();

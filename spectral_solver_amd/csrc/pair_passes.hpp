@@ -463,7 +463,7 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     // so Z_k and Z_{N-k} of a line sit in lanes tid and tid ^ NL of one
     // wavefront and the pair separation is a lane exchange instead of a trip
     // through an LDS tile with two barriers
-    constexpr bool MIRL = N <= 512 && Z::kMirrorableLast;
+    constexpr bool MIRL = Z::kMirrorableLast;
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int l = tid % NL;
