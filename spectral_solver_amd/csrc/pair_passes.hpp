@@ -447,17 +447,18 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
     constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
-    // ping-pong transforms; the pair separation then uses tile B, which the
-    // next transform writes only after its first barrier. Not at N = 1024,
-    // where the second tile's addressing pushes the 128-register x tile
-    // further into spills (measured 4% slower there, 3% faster at 512).
+    // ping-pong transforms; where the pair separation goes through LDS (no
+    // mirrored last stage) it uses tile B, which the next transform writes
+    // only after its first barrier. Not at N = 1024, where the second tile's
+    // addressing pushes the 128-register x tile further into spills (3%
+    // faster at 512; at 1024 the x pass takes 33.4 ms with it, 29.1 without)
     constexpr bool PP = N <= 512 && z_pingpong<T, N, LINES>();
     __shared__ cx<T> lds[z_lds<T, N, LINES>()];
     cx<T>* const sep = PP ? lds + Z::LDS : lds;
     // inverse transforms: first-stage butterflies in mirrored order, so the
     // two reads of each spectral row (elements k and N - k) come from one
-    // wavefront (ZTile::j_mirror); measured 2% faster at 512, 3% slower at
-    // 1024 (8-line tiles)
+    // wavefront (ZTile::j_mirror); measured 2% faster at 512, slower at 1024
+    // with its 8-line tiles (x pass 35.0 against 29.1 ms)
     constexpr bool MIR = N <= 512 && Z::kMirrorable;
     // forward transforms: last-stage butterflies in the same mirrored order,
     // so Z_k and Z_{N-k} of a line sit in lanes tid and tid ^ NL of one
