@@ -15,7 +15,9 @@ import spectral_solver_amd as sph
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(16, 16, 16), (12, 12, 12), (32, 16, 8), (8, 24, 48), (64, 64, 64), (96, 32, 16), (128, 8, 12),
-          (16, 8, 96), (24, 12, 24)]
+          (16, 8, 96), (24, 12, 24),
+          # the x and z passes' long plans (1024 = 8*4*4*8, 256 = 8*4*8) along x and z
+          (1024, 8, 12), (12, 8, 1024), (256, 16, 256)]
 
 
 def half_of(c2c):
