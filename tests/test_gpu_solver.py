@@ -300,3 +300,38 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
         den = sum(np.linalg.norm(oracle.c2c_to_half(o.u_hat(c))) ** 2 for c in range(3))
         print("  rel L2 vs oracle", np.sqrt(num / den), flush=True)
         assert np.sqrt(num / den) < match
+
+
+def read_csv_rows(path):
+    lines = open(path).read().strip().splitlines()
+    return lines[0], np.array([[float(v) for v in ln.split(",")] for ln in lines[1:]])
+
+
+@pytest.mark.parametrize("adaptive", [True, False])
+def test_driver_csv_matches_oracle_driver(tmp_path, adaptive):
+    """The GPU driver against the oracle's driver (impl.hpp:1310-1410) on the
+    same short run, row by row: same header, same output times (the adaptive
+    branch clips dt to t_end and to the next output time), diagnostics to
+    1e-12."""
+    print("case driver csv adaptive" if adaptive else "case driver csv fixed dt", flush=True)
+    kw = (dict(adaptive_cfl=True, dt=0.5, cfl=0.5, t_end=1.0, output_dt=0.25) if adaptive
+          else dict(adaptive_cfl=False, steps=20, dt=0.01, output_every=5))
+    gdir, odir = tmp_path / "gpu", tmp_path / "oracle"
+    gdir.mkdir()
+    odir.mkdir()
+    cfg = sph.RunConfig(n=32, reynolds=1600.0, output_dir=str(gdir), precision="fp64", print_every=1000, **kw)
+    if not adaptive:
+        cfg.t_end = kw["steps"] * kw["dt"]
+    sph.run_taylor_green(cfg)
+    _, _, steps = oracle.run_taylor_green(n=32, reynolds=1600.0, output_dir=str(odir), print_every=1000, **kw)
+    hg, rg = read_csv_rows(os.path.join(str(gdir), "taylor_green_diagnostics.csv"))
+    ho, ro = read_csv_rows(os.path.join(str(odir), "taylor_green_diagnostics.csv"))
+    print("rows", len(rg), "oracle steps", steps, flush=True)
+    assert hg == ho
+    assert rg.shape == ro.shape and len(rg) >= 5
+    assert np.abs(rg[:, 0] - ro[:, 0]).max() <= 1e-14  # output times
+    # E, dissipation, enstrophy, max |u|, dt (and with them the CFL columns)
+    # (the t = 0 row has dt = 0 in both)
+    for col in (1, 2, 3, 5, 6):
+        err, scale = np.abs(rg[:, col] - ro[:, col]), np.abs(ro[:, col])
+        assert np.all(err <= 1e-12 * scale), (hg.split(",")[col], (err / np.maximum(scale, 1e-300)).max())
