@@ -695,8 +695,9 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_div(const cx<T>* u0, 
 }
 
 // out[j] = reduce over blocks of partial[stride*b + j]; op bit j: 1 = max, 0 = sum
-static __global__ void __launch_bounds__(kReduceThreads) k_reduce_final(const double* partial, int nblocks, int stride,
-                                                                int max_mask, double* out) {
+// (launched from the solver's translation unit only)
+[[maybe_unused]] static __global__ void __launch_bounds__(kReduceThreads)
+    k_reduce_final(const double* partial, int nblocks, int stride, int max_mask, double* out) {
     __shared__ double red[kReduceThreads / 64];
     for (int j = 0; j < stride; ++j) {
         const bool is_max = (max_mask >> j) & 1;
