@@ -55,6 +55,40 @@ private:
     std::size_t n_ = 0;
 };
 
+// Pinned (page-locked) host memory from hipHostMalloc, for asynchronous
+// device-to-host copies.
+template <typename T>
+class PinnedHost {
+public:
+    explicit PinnedHost(std::size_t n) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    }
+    ~PinnedHost() {
+        if (p_) (void)hipHostFree(p_);
+    }
+    PinnedHost(const PinnedHost&) = delete;
+    PinnedHost& operator=(const PinnedHost&) = delete;
+    T* get() const { return p_; }
+
+private:
+    T* p_ = nullptr;
+};
+
+// An event without timing, for host waits on one point of a stream.
+class Event {
+public:
+    Event() { hip_check(hipEventCreateWithFlags(&e_, hipEventDisableTiming), "hipEventCreate"); }
+    ~Event() {
+        if (e_) (void)hipEventDestroy(e_);
+    }
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    hipEvent_t get() const { return e_; }
+
+private:
+    hipEvent_t e_ = nullptr;
+};
+
 // A non-blocking stream at the default priority.
 class Stream {
 public:

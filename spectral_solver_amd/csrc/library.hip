@@ -502,18 +502,22 @@ int sph_run_taylor_green(const sph_run_config* cfg_in, sph_diagnostics* final_ro
             while (t < cfg.t_end - 1.0e-14) {
                 if (step >= cfg.max_steps)
                     throw std::runtime_error("adaptive CFL run exceeded --max-steps before reaching --t-end");
-                const double vmax = S.max_velocity();
-                double main_dt = cfg.dt;
-                if (vmax > 1.0e-14) main_dt = std::min(main_dt, cfg.cfl * h / vmax);
-                const double main_cfl = vmax * main_dt / h;
-                double dt_step = std::min(main_dt, cfg.t_end - t);
-                if (next_output <= cfg.t_end + 1.0e-14) dt_step = std::min(dt_step, next_output - t);
+                // dt from max |u| of the current state (the step's own x pass
+                // reduces it: TaylorGreenSolver::step_cfl)
+                double main_dt = cfg.dt, main_cfl = 0.0, dt_step = 0.0, actual_cfl = 0.0;
+                S.step_cfl([&](double vmax) {
+                    main_dt = cfg.dt;
+                    if (vmax > 1.0e-14) main_dt = std::min(main_dt, cfg.cfl * h / vmax);
+                    main_cfl = vmax * main_dt / h;
+                    dt_step = std::min(main_dt, cfg.t_end - t);
+                    if (next_output <= cfg.t_end + 1.0e-14) dt_step = std::min(dt_step, next_output - t);
+                    actual_cfl = vmax * dt_step / h;
+                    return dt_step <= 1.0e-14 ? 0.0 : dt_step;
+                });
                 if (dt_step <= 1.0e-14) {
                     next_output += cfg.output_dt;
                     continue;
                 }
-                const double actual_cfl = vmax * dt_step / h;
-                S.step(dt_step);
                 t += dt_step;
                 ++step;
                 if (t >= next_output - 1.0e-12 || t >= cfg.t_end - 1.0e-12) {

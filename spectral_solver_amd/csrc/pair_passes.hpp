@@ -428,7 +428,22 @@ struct PairXArgs {
     T scale;
     const cx<T>* tw;
     long long ms_w;
+    // if set (register-direct form): max |u|^2 over the grid points of the
+    // unshifted velocity, as the bits of non-negative doubles (whose order is
+    // that of the unsigned integers) combined with atomicMax into
+    // kVmaxSlots slots kVmaxStride apart (one 128-byte line each: one
+    // address for all wavefronts serialises them, measured 1.1 ms of a 2.5 ms
+    // pass at 512^3); the maximum is that of the slots. Zero them first.
+    unsigned long long* vmax2;
 };
+constexpr int kVmaxSlots = 256;
+constexpr int kVmaxStride = 16;
+
+// the kVmaxSlots slots = 0 on the stream (a kernel, so that a captured graph
+// holds no memset; launched from the solver's translation unit only)
+[[maybe_unused]] static __global__ void k_zero_vmax(unsigned long long* p) {
+    if (threadIdx.x < kVmaxSlots) p[threadIdx.x * kVmaxStride] = 0ull;
+}
 
 // Register-direct form: the x lines are transformed with the z-pass machinery
 // (zfft), lanes over y. The inverse first stage loads its rows straight from
@@ -438,7 +453,9 @@ struct PairXArgs {
 // only the forward output goes through LDS once more, for the pair separation
 // of Z_k and Z_{N-k}. Same formulas as the LDS-staged form below (results
 // agree to rounding; the compiler may contract multiply-adds differently).
-template <typename T, int N, int LINES = 0>
+// VMAX: also reduce max |u|^2 of the physical velocity into a.vmax2 (the
+// adaptive time step's CFL condition, without transforms of its own).
+template <typename T, int N, int LINES = 0, bool VMAX = false>
 // The second bound is the minimum number of waves per SIMD: 512-thread tiles
 // ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
 // at 128 like the 1024-thread form.
@@ -528,6 +545,20 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 u[c][i + r * TLS] = w[0][i][r].x * a.scale;
                 s[c][i + r * TLS] = w[0][i][r].y * a.scale;
             }
+    }
+    if constexpr (VMAX) {
+        // |u|^2 in double as the diagnostics' reduction forms it (k_reduce_phys);
+        // lanes past the last line hold zeros
+        double m = 0.0;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const double p = u[0][i], q = u[1][i], r = u[2][i];
+            m = fmax(m, p * p + q * q + r * r);
+        }
+        m = wave_max(m);
+        const unsigned slot = (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % kVmaxSlots;
+        if (tid % 64 == 0)
+            atomicMax(a.vmax2 + slot * kVmaxStride, static_cast<unsigned long long>(__double_as_longlong(m)));
     }
     constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
     constexpr int PB[6] = {0, 1, 2, 1, 2, 2};

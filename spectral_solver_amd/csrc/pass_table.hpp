@@ -131,8 +131,13 @@ struct Launch {
     static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
         const dim3 grid((ny + L - 1) / L, nz, members);
-        k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
+        if (a.vmax2)
+            k_px_products_rd<T, N, L, true><<<grid, ZV::NT, 0, s>>>(a);
+        else
+            k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
     }
+    // The register-direct variants also reduce the velocity maximum when
+    // a.vmax2 is set; the LDS-staged one does not (x_reduces_velocity).
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
         // the register-direct form addresses rows of at most N * N modes
         if (static_cast<long long>(ny) * nz > static_cast<long long>(N) * N) variant = 2;

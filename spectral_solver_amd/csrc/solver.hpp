@@ -13,7 +13,9 @@
 #include <complex>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -237,7 +239,10 @@ public:
     // stage `stage` for members [first, first + members); u and g address
     // member 0 of the state family. Pass 0 runs only when the velocity pairs
     // in W are not those of the state (pairs_valid()); pass 4 leaves them so.
-    void launch_pass(int k, int stage, cx<T>* u, cx<T>* g, int first, int members, hipStream_t s) const {
+    // vmax2 (pass 2 only, register-direct x pass): max |u|^2 of the state is
+    // reduced into it as well.
+    void launch_pass(int k, int stage, cx<T>* u, cx<T>* g, int first, int members, hipStream_t s,
+                     unsigned long long* vmax2 = nullptr) const {
         const std::int64_t M2 = 2 * modes;
         cx<T>* W = W_.get() + first * sw;
         u += first * su;
@@ -258,6 +263,7 @@ public:
                 a.scale = T(1) / static_cast<T>(npts);
                 a.tw = tw_.get();
                 a.ms_w = sw;
+                a.vmax2 = vmax2;
                 pt_->px_products(a, n, n, members, var_[1], s);
                 break;
             }
@@ -298,6 +304,25 @@ public:
     // passes 1-4 of a stage (prepare_pairs first)
     void launch_stage(int stage, hipStream_t s) const {
         for (int k = 1; k < kPasses; ++k) launch_pass(k, stage, U_.get(), G_.get(), 0, count, s);
+    }
+
+    // A step split at the first use of dt, for the adaptive time step: the
+    // head is stage 0's inverse y and x passes, which need no dt and whose x
+    // pass forms the physical velocity of the current state anyway, so it
+    // also reduces max |u|^2 into *vmax2; stage 0's forward y pass needs no
+    // dt either (launch_step_mid); the tail is the rest of the step, from
+    // stage 0's forward z and update on. One solve per group, register-direct
+    // x pass only.
+    bool x_reduces_velocity() const { return count == 1 && var_[1] != 2; }
+    void launch_step_head(unsigned long long* vmax2, hipStream_t s) const {
+        k_zero_vmax<<<1, kVmaxSlots, 0, s>>>(vmax2);
+        launch_pass(1, 0, U_.get(), G_.get(), 0, 1, s);
+        launch_pass(2, 0, U_.get(), G_.get(), 0, 1, s, vmax2);
+    }
+    void launch_step_mid(hipStream_t s) const { launch_pass(3, 0, U_.get(), G_.get(), 0, 1, s); }
+    void launch_step_tail(hipStream_t s) const {
+        launch_pass(4, 0, U_.get(), G_.get(), 0, 1, s);
+        for (int st = 1; st < 3; ++st) launch_stage(st, s);
     }
 
     // member m: u_hat -> physical (the group's one physical buffer).
@@ -651,6 +676,9 @@ class SolverBase {
 public:
     virtual ~SolverBase() = default;
     virtual void step(double dt) = 0;
+    // One step whose dt = dt_of(max |u| of the current state); dt_of returns
+    // a value <= 0 to take no step. Returns the dt used (or that value).
+    virtual double step_cfl(const std::function<double(double)>& dt_of) = 0;
     virtual sph_diagnostics diagnostics(double t) = 0;
     virtual double max_velocity() = 0;
     virtual void get_uhat(int c, double* host) = 0;
@@ -696,6 +724,56 @@ public:
         prof_.backward_calls += 18;
     }
 
+    // max |u| comes from stage 0's x pass (Group::launch_step_head) instead
+    // of transforms of its own; the same physical field, formed by the x
+    // pass's transforms (equal to max_velocity() to rounding). The host waits
+    // for that value only: stage 0's forward y pass is queued behind its
+    // copy, so the dt choice and the tail's launch overlap it on the GPU.
+    double step_cfl(const std::function<double(double)>& dt_of) override {
+        const hipStream_t s = stream_.get();
+        if (!g_.x_reduces_velocity()) {
+            const double dt = dt_of(max_velocity());
+            if (dt > 0.0) step(dt);
+            return dt;
+        }
+        auto ev = timer_.begin(s);
+        g_.prepare_pairs(s);
+        g_.launch_step_head(vmax2_.get(), s);
+        hip_check(hipMemcpyAsync(vmax2_host_.get(), vmax2_.get(), kVmaxSlots * kVmaxStride * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, s),
+                  "hipMemcpy vmax");
+        hip_check(hipEventRecord(vmax_ready_.get(), s), "hipEventRecord");
+        g_.launch_step_mid(s);
+        hip_check(hipGetLastError(), "TaylorGreenSolver::step_cfl");
+        hip_check(hipEventSynchronize(vmax_ready_.get()), "step_cfl");
+        unsigned long long bits = 0;
+        for (int k = 0; k < kVmaxSlots; ++k) bits = std::max(bits, vmax2_host_.get()[k * kVmaxStride]);
+        double vmax2 = 0.0;
+        std::memcpy(&vmax2, &bits, sizeof(vmax2));
+        prof_.max_velocity_calls += 1;
+        prof_.backward_calls += 3;
+        const double dt = dt_of(std::sqrt(vmax2));
+        if (!(dt > 0.0)) {
+            // the head and the forward y pass consumed the velocity pairs; no
+            // step is taken
+            g_.invalidate_pairs();
+            timer_.end(ev, s);
+            return dt;
+        }
+        g_.set_dt(&dt, s);
+        if (use_graph_)
+            tail_graph_.launch(s, [&] { g_.launch_step_tail(s); });
+        else
+            g_.launch_step_tail(s);
+        hip_check(hipGetLastError(), "TaylorGreenSolver::step_cfl");
+        timer_.end(ev, s);
+        prof_.step_calls += 1;
+        prof_.rhs_calls += 3;
+        prof_.forward_calls += 36;
+        prof_.backward_calls += 18;
+        return dt;
+    }
+
     sph_diagnostics diagnostics(double t) override {
         const auto t0 = std::chrono::steady_clock::now();
         const sph_diagnostics d = g_.diagnostics(0, t, stream_.get());
@@ -731,6 +809,13 @@ private:
     Group<T> g_;
     bool use_graph_ = true;
     StepGraph graph_;
+    // step_cfl: the rest of a step after its forward y pass of stage 0, the
+    // max |u|^2 slots on the device and their pinned host copy, the copy's
+    // event
+    StepGraph tail_graph_;
+    DeviceArray<unsigned long long> vmax2_{kVmaxSlots * kVmaxStride};
+    PinnedHost<unsigned long long> vmax2_host_{kVmaxSlots * kVmaxStride};
+    Event vmax_ready_;
     StepTimer timer_;
     sph_profile prof_{};
 };
