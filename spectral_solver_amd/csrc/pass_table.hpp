@@ -35,6 +35,9 @@ struct PassTable {
                    const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
     void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, hipStream_t);
+    // the register-direct x pass of one cubic solve that also reduces max
+    // |u|^2 into a.vmax2 (variants 0 and 1; px_products_vmax)
+    void (*px_products_vmax)(const PairXArgs<T>&, int variant, hipStream_t);
     // forward z + update, then the next stage's inverse z (k_pfwd_z_rk_inv)
     void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, hipStream_t);
 };
@@ -42,6 +45,13 @@ struct PassTable {
 // nullptr when n is not a compiled length.
 template <typename T>
 const PassTable<T>* pass_table(int n);
+
+// Defined in passes_vmax.hip, a translation unit of its own: compiled in
+// the same module as the plain x pass, its kernels changed that pass's
+// register allocation (a 12-byte spill per thread at 512^3, 0.13 GB more
+// HBM writes per launch).
+template <typename T, int N>
+void px_products_vmax(const PairXArgs<T>& a, int variant, hipStream_t s);
 
 // The compiled transform lengths.
 #define SPH_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
@@ -131,13 +141,8 @@ struct Launch {
     static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
         const dim3 grid((ny + L - 1) / L, nz, members);
-        if (a.vmax2)
-            k_px_products_rd<T, N, L, true><<<grid, ZV::NT, 0, s>>>(a);
-        else
-            k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
+        k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
     }
-    // The register-direct variants also reduce the velocity maximum when
-    // a.vmax2 is set; the LDS-staged one does not (x_reduces_velocity).
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
         // the register-direct form addresses rows of at most N * N modes
         if (static_cast<long long>(ny) * nz > static_cast<long long>(N) * N) variant = 2;
@@ -169,6 +174,7 @@ struct Launch {
                                         &plines,
                                         &pinv_x,
                                         &px_products,
+                                        &px_products_vmax<T, N>,
                                         &pfwd_z_rk};
 };
 

@@ -264,7 +264,10 @@ public:
                 a.tw = tw_.get();
                 a.ms_w = sw;
                 a.vmax2 = vmax2;
-                pt_->px_products(a, n, n, members, var_[1], s);
+                if (vmax2)
+                    pt_->px_products_vmax(a, var_[1], s);
+                else
+                    pt_->px_products(a, n, n, members, var_[1], s);
                 break;
             }
             case 3:
