@@ -9,9 +9,12 @@ two runs is compared after the timed steps.
 
 Prints one JSON line. `value` is Mpts/s = n^3 / (seconds per RK3 step) / 1e6.
 `roofline` describes the dominant pass kernel: algorithmic HBM bytes per launch
-divided by its mean launch time (HIP events on the solver's stream).
-`cpu_baseline` times the CPU oracle (a C++ restatement of the reference's CPU
-path) on one RK3 stage of the same 512^3 FP32 workload on the host cores.
+divided by its mean duration inside real steps (HIP events between direct
+launches on the solver's stream; the isolated relaunch figure is reported
+beside it). `cpu_baseline` times the CPU oracle (a C++ restatement of the
+reference's CPU path) on one full RK3 step of the same 512^3 FP32 workload on
+the host cores, and on the reference's 64^3 FP64 configuration on 1 thread and
+on all cores.
 """
 from __future__ import annotations
 
@@ -197,16 +200,63 @@ def fft_vs_rocfft(sph, n: int, reps: int = 10):
                     "the 1/N^3 scaling in ours, not in rocFFT's"}
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n: int, dt: float):
+    """The CPU oracle (C++ restatement of the reference's AVA(CPU)+FFTW path,
+    C2C storage, its own Stockham FFT in place of FFTW) timed on this host:
+    steady_clock around step_rk3 only, as the reference's step timer
+    (src/ava_taylor_green_impl.hpp:980,1043), after one untimed step.
+
+    value: one full RK3 step of the benchmark workload (n^3 FP32) on all
+    cores. Beside it, the reference path's own configuration: 64^3 FP64 on 1
+    thread (the reference's serial build) and on all cores, next to the
+    published 64^3 FP64 CPU steps, and one 256^3 FP64 step on all cores."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"cpu baseline: oracle {n}^3 FP32, one RK3 stage, {threads} threads")
-    o = oracle.TaylorGreenOracle(n, 1600.0, single=True)
-    sec = o.first_stage_seconds(dt)
-    o.close()
-    return {"value": round(n ** 3 / (3.0 * sec) / 1e6, 4), "unit": "Mpts/s", "cores": threads, "kind": "port",
-            "sample": f"one RK3 stage (of 3) of the {n}^3 FP32 Taylor-Green step, C2C oracle, {sec:.2f} s, "
-                      f"scaled x3 to a step"}
+
+    def per_step(size, single, nthreads, steps):
+        oracle.set_threads(nthreads)
+        o = oracle.TaylorGreenOracle(size, 1600.0, single=single)
+        o.step_rk3(dt)
+        t0 = o.step_seconds()
+        for _ in range(steps):
+            o.step_rk3(dt)
+        sec = (o.step_seconds() - t0) / steps
+        o.close()
+        return sec
+
+    log(f"cpu baseline: oracle on {threads} threads ({cpu_model()}, nproc {os.cpu_count()})")
+    sec64_1 = per_step(64, False, 1, 3)
+    sec64_all = per_step(64, False, threads, 5)
+    log(f"cpu baseline: 64^3 FP64 {sec64_1 * 1e3:.1f} ms/step (1 thread), {sec64_all * 1e3:.1f} ms/step "
+        f"({threads} threads)")
+    sec256 = per_step(256, False, threads, 1)
+    log(f"cpu baseline: 256^3 FP64 {sec256:.2f} s/step ({threads} threads)")
+    sec = per_step(n, True, threads, 1)
+    log(f"cpu baseline: {n}^3 FP32 {sec:.2f} s/step ({threads} threads)")
+    oracle.set_threads(threads)
+    mp = lambda size, t: round(size ** 3 / t / 1e6, 4)
+    return {"value": mp(n, sec), "unit": "Mpts/s", "cores": threads, "kind": "port",
+            "sample": f"one full RK3 step of the {n}^3 FP32 Taylor-Green workload after one untimed step, "
+                      f"{sec:.2f} s (C2C oracle: the reference's algorithm with its own FFT in place of FFTW)",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "fp64_64cube": {"ms_per_step_1_thread": round(sec64_1 * 1e3, 2),
+                            f"ms_per_step_{threads}_threads": round(sec64_all * 1e3, 2),
+                            "mpts_per_s_1_thread": mp(64, sec64_1),
+                            "published_reference_ms_per_step": {
+                                "i9-12900H, serial build, FFTW": 188.1, "i5-13500, serial build, FFTW": 106.3},
+                            "note": "the published figures are the reference's AVA(CPU)+FFTW path (Benchmark.md:33,"
+                                    "42,172,280); the oracle's FFT is a plain mixed-radix Stockham, not FFTW"},
+            "fp64_256cube": {f"s_per_step_{threads}_threads": round(sec256, 3), "steps_timed": 1}}
 
 
 def accuracy_run(sph, n: int, dt: float, t_end: float, every: float):
@@ -250,7 +300,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--dt", type=float, default=5e-4)
-    ap.add_argument("--reps", type=int, default=5, help="launches per pass for the kernel timing")
+    ap.add_argument("--reps", type=int, default=5, help="launches per pass for the isolated kernel timing")
+    ap.add_argument("--instep-steps", type=int, default=3,
+                    help="real steps whose pass kernels are timed with events for the roofline")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -277,6 +329,9 @@ def main():
 
     log(f"fp32 {n}^3: warmup {W}, timing {K} steps")
     s32, sec32 = time_steps(sph, n, "fp32", K, W, args.dt)
+    # diagnostics after the timed steps (the in-step pass timing below
+    # advances the solver further)
+    d32_timed = s32.diagnostics((W + K) * args.dt)
     check = None
     if args.check_n:
         # the Taylor-Green field stays spectrally resolved on a coarser grid
@@ -294,8 +349,13 @@ def main():
                  "rel_err": {k: abs(getattr(d32, k) - getattr(d64, k)) / abs(getattr(d64, k))
                              for k in ("kinetic_energy", "dissipation", "enstrophy")}}
         log("check:", check)
+    # pass kernels inside real steps (direct launches, HIP events between
+    # them on the solver's stream), then the same passes relaunched in
+    # isolation (sph_solver_time_passes) for comparison
+    instep = s32.time_step_passes(args.dt, args.instep_steps)
+    log("in-step pass ms:", {k: round(v[0], 4) if isinstance(v, tuple) else round(v, 4) for k, v in instep.items()})
     passes = s32.time_passes(args.reps)
-    log("pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
+    log("isolated pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
     tuning = s32.tuning()
     log("autotuner (variant, lines per tile, ms per variant):", tuning)
 
@@ -305,7 +365,7 @@ def main():
         log(f"fp64 {n}^3: warmup {W}, timing {K} steps")
         s64, sec64 = time_steps(sph, n, "fp64", K, W, args.dt)
         t = (W + K) * args.dt
-        d32, d64 = s32.diagnostics(t), s64.diagnostics(t)
+        d32, d64 = d32_timed, s64.diagnostics(t)
         eps_err = abs(d32.dissipation - d64.dissipation) / d64.dissipation
         e_err = abs(d32.kinetic_energy - d64.kinetic_energy) / d64.kinetic_energy
         z_err = abs(d32.enstrophy - d64.enstrophy) / d64.enstrophy
@@ -314,19 +374,27 @@ def main():
                 "roofline_frac": round(step_bytes(n, 8) / sec64 / 1e9 / HBM_PEAK_GBS, 4)}
         s64.close()
 
-    dom = max(range(len(PASS_NAMES)), key=lambda k: passes[PASS_NAMES[k]][0])
+    in_step = [k for k in range(1, len(PASS_NAMES))]  # the standalone inverse z runs only after set-up
+    dom = max(in_step, key=lambda k: instep[PASS_NAMES[k]][0])
     kernel = KERNEL_NAMES[dom]
     if PASS_NAMES[dom] == "x_products" and tuning["x_products"][0] == 2:
         kernel = "k_px_products"  # the LDS-staged form
-    ms, by = passes[PASS_NAMES[dom]]
+    ms, by = instep[PASS_NAMES[dom]]
+    iso_ms = passes[PASS_NAMES[dom]][0]
     achieved = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel, n),
                 "kernel": f"{PASS_NAMES[dom]} ({kernel})", "launch_ms": round(ms, 4),
+                "launch_ms_source": f"mean over the {3 * args.instep_steps} stages of {args.instep_steps} real steps, "
+                                    "HIP events between direct launches on the solver's stream",
+                "isolated_launch_ms": round(iso_ms, 4),
+                "isolated_frac": round(by / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "algorithmic_bytes": by,
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "step_frac_implemented_bytes": round(implemented_step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
-                "passes_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()},
+                "passes_in_step_ms": {k: round(instep[k][0], 4) for k in PASS_NAMES[1:]},
+                "passes_in_step_gbs": {k: round(instep[k][1] / (instep[k][0] * 1e-3) / 1e9, 1) for k in PASS_NAMES[1:]},
+                "passes_isolated_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()},
                 "autotuned": {k: {"variant": v[0], "lines": v[1]} for k, v in tuning.items()}}
     s32.close()
 

@@ -162,6 +162,25 @@ int sph_solver_synchronize(sph_solver_t solver);
  * unchanged. */
 int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* bytes);
 
+/* The pass kernels inside real steps: `steps` RK3 steps of dt with the
+ * kernels launched directly (not through the graph) and a HIP event between
+ * consecutive launches on the solver's stream. ms[6]: [0] the standalone
+ * inverse z if the velocity pairs had to be formed first (else 0), [1..4] the
+ * mean duration of inverse-y, x, forward-y and forward-z + update + next
+ * inverse z over the steps' 3*steps stages, [5] the mean step time;
+ * bytes[5]: algorithmic HBM bytes per launch as sph_solver_time_passes.
+ * The state advances by `steps` steps (as sph_solver_step_rk3 would). */
+int sph_solver_time_step_passes(sph_solver_t solver, double dt, int steps, double* ms, double* bytes);
+
+/* One adaptive RK3 step as run_taylor_green's CFL loop takes it
+ * (impl.hpp:1363-1378, without the clipping to output times):
+ * dt = min(dt_max, cfl * (2 pi / n) / max|u|) (dt_max when max|u| <= 1e-14),
+ * max|u| of the state before the step. Where the x pass's fused reduction
+ * runs (*fused = 1; may be NULL), max|u| comes from the step's own stage-0 x
+ * pass, otherwise from max_velocity(). Returns max|u| and the dt taken. */
+int sph_solver_step_adaptive(sph_solver_t solver, double dt_max, double cfl, double* max_velocity, double* dt,
+                             int* fused);
+
 /* The choices of the solver's autotuner (the reference plans its FFTs with
  * FFTW_ESTIMATE / cuFFT, impl.hpp:257-412; here the y and x passes come in
  * several tile widths and kernel forms, timed at creation on the solver's own

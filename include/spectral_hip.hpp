@@ -25,17 +25,56 @@
 #include <string>
 #include <vector>
 
+#if defined(SPECTRAL_HIP_WITH_MPI) && !defined(MPI_VERSION)
+#include <mpi.h>
+#endif
+
 namespace spectral {
 
 inline void sph_check(int rc) {
     if (rc != 0) throw std::runtime_error(sph_last_error());
 }
 
-// The reference passes an MPI communicator; this single-GPU backend runs on
-// one rank, so the communicator is a placeholder accepted for signature
-// compatibility.
+// Communicator. The reference's backends take and return an MPI_Comm
+// (src/fftw_fft.hpp:15,26) and its solver hands fft.comm() to MPI_Allreduce
+// (src/spectral_solver.cpp:317,334). With MPI (<mpi.h> included before this
+// header, or SPECTRAL_HIP_WITH_MPI defined and <mpi.h> on the include path),
+// Comm is MPI_Comm for any MPI implementation (an int under MPICH, a pointer
+// under Open MPI) and kCommSelf is MPI_COMM_SELF; the backend runs on one rank
+// like the serial FFTW backend. Without MPI, Comm is a one-rank stand-in.
+#ifdef MPI_VERSION
+using Comm = MPI_Comm;
+inline const Comm kCommSelf = MPI_COMM_SELF;
+inline int communicator_size(Comm comm) {
+    int initialized = 0;
+    MPI_Initialized(&initialized);
+    int size = 1;
+    if (initialized) MPI_Comm_size(comm, &size);
+    return size;
+}
+inline double allreduce_sum(double v, Comm comm) {
+    int initialized = 0;
+    MPI_Initialized(&initialized);
+    if (!initialized) return v;
+    double out = 0.0;
+    MPI_Allreduce(&v, &out, 1, MPI_DOUBLE, MPI_SUM, comm);
+    return out;
+}
+inline double allreduce_max(double v, Comm comm) {
+    int initialized = 0;
+    MPI_Initialized(&initialized);
+    if (!initialized) return v;
+    double out = 0.0;
+    MPI_Allreduce(&v, &out, 1, MPI_DOUBLE, MPI_MAX, comm);
+    return out;
+}
+#else
 using Comm = int;
 constexpr Comm kCommSelf = 0;
+inline int communicator_size(Comm) { return 1; }
+inline double allreduce_sum(double v, Comm) { return v; }
+inline double allreduce_max(double v, Comm) { return v; }
+#endif
 
 struct SpectralMode {
     double kx = 0.0;
@@ -52,7 +91,8 @@ public:
     HipFft3D(std::array<int, 3> shape, std::array<double, 3> lengths, Comm comm = kCommSelf,
              int precision = SPH_FP64)
         : shape_(shape), lengths_(lengths) {
-        (void)comm;
+        // one rank, as the serial FFTW backend requires (src/fftw_fft.cpp:10-14)
+        if (communicator_size(comm) != 1) throw std::runtime_error("The HIP backend supports one MPI rank");
         sph_check(sph_fft_create(shape_.data(), lengths_.data(), precision, &h_));
         int64_t np = 0, nm = 0;
         sph_check(sph_fft_sizes(h_, &np, &nm));
@@ -309,10 +349,11 @@ public:
         return u;
     }
 
+    // sums and maxima over the ranks of fft().comm() (spectral_solver.cpp:311-336)
     double kinetic_energy(const VectorPhysical& u) const {
         double s = 0.0;
         for (std::size_t i = 0; i < u[0].size(); ++i) s += 0.5 * (u[0][i] * u[0][i] + u[1][i] * u[1][i] + u[2][i] * u[2][i]);
-        return s / static_cast<double>(fft_.global_physical_size());
+        return allreduce_sum(s, fft_.comm()) / static_cast<double>(fft_.global_physical_size());
     }
     double kinetic_energy_spectral(const VectorSpectral& u_hat) { return kinetic_energy(to_physical(u_hat)); }
     double max_velocity(const VectorSpectral& u_hat) {
@@ -320,7 +361,7 @@ public:
         double m = 0.0;
         for (std::size_t i = 0; i < u[0].size(); ++i)
             m = std::max(m, std::sqrt(u[0][i] * u[0][i] + u[1][i] * u[1][i] + u[2][i] * u[2][i]));
-        return m;
+        return allreduce_max(m, fft_.comm());
     }
     VectorSpectral vorticity_hat(const VectorSpectral& u_hat) const {
         VectorSpectral w;
@@ -338,7 +379,7 @@ public:
         const auto w = to_physical(vorticity_hat(u_hat));
         double s = 0.0;
         for (std::size_t i = 0; i < w[0].size(); ++i) s += w[0][i] * w[0][i] + w[1][i] * w[1][i] + w[2][i] * w[2][i];
-        return 0.5 * s / static_cast<double>(fft_.global_physical_size());
+        return 0.5 * allreduce_sum(s, fft_.comm()) / static_cast<double>(fft_.global_physical_size());
     }
     double dissipation(const VectorSpectral& u_hat) { return 2.0 * nu_ * enstrophy(u_hat); }
     double max_spectral_divergence(const VectorSpectral& u_hat) const {
@@ -346,7 +387,7 @@ public:
         fft_.for_each_spectral([&](std::size_t id, const SpectralMode& md) {
             m = std::max(m, std::abs(md.kx * at(u_hat[0], id) + md.ky * at(u_hat[1], id) + md.kz * at(u_hat[2], id)));
         });
-        return m;
+        return allreduce_max(m, fft_.comm());
     }
     Diagnostics diagnostics(const VectorSpectral& u_hat, double t) {
         const double ens = enstrophy(u_hat);

@@ -52,6 +52,9 @@ def lib():
             ctypes.POINTER(ctypes.c_ubyte), dp]
         L.orc_dealiased_product.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_solver_first_stage.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.orc_legacy_run.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, dp, dp, dp, dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -154,6 +157,37 @@ def run_taylor_green(n=64, steps=80, max_steps=10000000, output_every=1, print_e
                                       dt, cfl, t_end, output_dt, reynolds, 1 if single else 0,
                                       output_dir.encode(), _dptr(row), cnt, ctypes.byref(st)))
     return dict(zip(DIAG_KEYS, row.tolist())), dict(zip(COUNTER_KEYS, list(cnt))), st.value
+
+
+def set_threads(threads: int) -> None:
+    """Threads of the oracle's OpenMP loops (1 = the reference's serial build)."""
+    _check(lib().orc_set_threads(int(threads)))
+
+
+def legacy_run(shape, lengths, nu: float, dealias: bool, steps: int, dt: float, phys=None):
+    """Restatement of the reference's legacy SpectralNavierStokes
+    (src/spectral_solver.cpp:35-409, legacy.hpp): physical velocity `phys`
+    (three real [nz, ny, nx] arrays; None = its taylor_green(1, 1)) ->
+    to_spectral, project, truncate, then `steps` RK3 steps of dt.
+
+    Returns (diagnostics [(steps+1), 5] rows t, E, dissipation, enstrophy,
+    max divergence; nonlinear_rhs of the initial state; final state), the
+    last two as three complex C2C [nz, ny, nx] arrays each."""
+    nx, ny, nz = (int(v) for v in shape)
+    npts = nx * ny * nz
+    shp = (ctypes.c_int * 3)(nx, ny, nz)
+    lens = np.ascontiguousarray(lengths, np.float64)
+    pin = None
+    if phys is not None:
+        pin = np.ascontiguousarray(np.stack([np.asarray(p, np.float64).reshape(nz, ny, nx) for p in phys]))
+    diag = np.zeros((steps + 1, 5))
+    rhs0 = np.zeros((3, nz, ny, nx), np.complex128)
+    state = np.zeros((3, nz, ny, nx), np.complex128)
+    _check(lib().orc_legacy_run(shp, _dptr(lens), float(nu), 1 if dealias else 0, int(steps), float(dt),
+                                _dptr(pin) if pin is not None else None, _dptr(diag),
+                                _dptr(rhs0.view(np.float64)), _dptr(state.view(np.float64))))
+    assert npts == state[0].size
+    return diag, list(rhs0), list(state)
 
 
 def mode_tables(n: int):

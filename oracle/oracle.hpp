@@ -99,6 +99,8 @@ public:
             }
             ns *= r;
         }
+        twb_.resize(tw_.size());
+        for (std::size_t i = 0; i < tw_.size(); ++i) twb_[i] = {tw_[i].x, -tw_[i].y};
         for (int r : radix_) {
             std::vector<Cx<Real>> roots(r);
             for (int q = 0; q < r; ++q) {
@@ -113,32 +115,62 @@ public:
 
     // sign = -1 forward, +1 backward (unnormalised). `a` holds the input and
     // receives the output; `work` must hold n elements.
-    void exec(Cx<Real>* a, Cx<Real>* work, int sign) const {
+    void exec(Cx<Real>* a, Cx<Real>* work, int sign) const { exec_batch(a, work, 1, 1, sign); }
+
+    // The same transform of `count` lines stored element-major with stride B
+    // between elements (line b at a[e*B + b], count <= B): every element goes
+    // through exactly the operations of exec, lines side by side, so the
+    // results are those of exec line by line. `work` must hold n*B elements.
+    void exec_batch(Cx<Real>* a, Cx<Real>* work, int B, int count, int sign) const {
         Cx<Real>* src = a;
         Cx<Real>* dst = work;
         int ns = 1;
+        const std::vector<Cx<Real>>& tws = sign > 0 ? twb_ : tw_;
         for (std::size_t s = 0; s < radix_.size(); ++s) {
             const int r = radix_[s];
             const int m = n_ / r;
-            const Cx<Real>* tw = tw_.data() + stage_offset_[s];
-            Cx<Real> x[64];
+            const Cx<Real>* tw = tws.data() + stage_offset_[s];
             for (int j = 0; j < m; ++j) {
                 const int k = j % ns;
-                for (int q = 0; q < r; ++q) {
-                    Cx<Real> v = src[j + q * m];
-                    Cx<Real> w = tw[k * r + q];
-                    if (sign > 0) w.y = -w.y;
-                    x[q] = q == 0 ? v : cx_mul(v, w);
-                }
-                butterfly(x, r, s, sign);
+                const Cx<Real>* w = tw + k * r;
                 const int base = (j / ns) * ns * r + k;
-                for (int q = 0; q < r; ++q) dst[base + q * ns] = x[q];
+                const Cx<Real>* in = src + static_cast<std::size_t>(j) * B;
+                Cx<Real>* out = dst + static_cast<std::size_t>(base) * B;
+                const std::size_t is = static_cast<std::size_t>(m) * B, os = static_cast<std::size_t>(ns) * B;
+                if (r == 4) {
+                    for (int b = 0; b < count; ++b) {
+                        const Cx<Real> x0 = in[b], x1 = cx_mul(in[is + b], w[1]), x2 = cx_mul(in[2 * is + b], w[2]),
+                                       x3 = cx_mul(in[3 * is + b], w[3]);
+                        const Cx<Real> p = cx_add(x0, x2), q = cx_sub(x0, x2);
+                        const Cx<Real> c = cx_add(x1, x3), d = cx_sub(x1, x3);
+                        const Cx<Real> dr = sign < 0 ? Cx<Real>{d.y, -d.x} : Cx<Real>{-d.y, d.x};
+                        out[b] = cx_add(p, c);
+                        out[2 * os + b] = cx_sub(p, c);
+                        out[os + b] = cx_add(q, dr);
+                        out[3 * os + b] = cx_sub(q, dr);
+                    }
+                } else if (r == 2) {
+                    for (int b = 0; b < count; ++b) {
+                        const Cx<Real> x0 = in[b], x1 = cx_mul(in[is + b], w[1]);
+                        out[b] = cx_add(x0, x1);
+                        out[os + b] = cx_sub(x0, x1);
+                    }
+                } else {
+                    Cx<Real> x[64];
+                    for (int b = 0; b < count; ++b) {
+                        x[0] = in[b];
+                        for (int q = 1; q < r; ++q) x[q] = cx_mul(in[q * is + b], w[q]);
+                        butterfly(x, r, s, sign);
+                        for (int q = 0; q < r; ++q) out[q * os + b] = x[q];
+                    }
+                }
             }
             std::swap(src, dst);
             ns *= r;
         }
         if (src != a) {
-            for (int i = 0; i < n_; ++i) a[i] = src[i];
+            for (int e = 0; e < n_; ++e)
+                for (int b = 0; b < count; ++b) a[static_cast<std::size_t>(e) * B + b] = src[static_cast<std::size_t>(e) * B + b];
         }
     }
 
@@ -176,7 +208,7 @@ private:
     int n_;
     std::vector<int> radix_;
     std::vector<int> stage_offset_;
-    std::vector<Cx<Real>> tw_;
+    std::vector<Cx<Real>> tw_, twb_;  // forward twiddles and their conjugates
     std::vector<std::vector<Cx<Real>>> roots_;
 };
 
@@ -188,22 +220,32 @@ public:
 
     std::size_t size() const { return static_cast<std::size_t>(nx_) * ny_ * nz_; }
 
+    // Lines of each axis go through Line1d::exec_batch kBlock at a time,
+    // gathered element-major into a per-thread buffer.
     void exec(Cx<Real>* a, int sign) const {
         const int nx = nx_, ny = ny_, nz = nz_;
-        // x lines: contiguous.
+        constexpr int kBlock = 16;
+        // x lines: contiguous rows; a block is kBlock consecutive rows
+        const long rows = static_cast<long>(ny) * nz;
 #pragma omp parallel
         {
-            std::vector<Cx<Real>> work(nx);
+            std::vector<Cx<Real>> buf(static_cast<std::size_t>(nx) * kBlock), work(buf.size());
 #pragma omp for schedule(static)
-            for (long line = 0; line < static_cast<long>(ny) * nz; ++line) {
-                fx_.exec(a + line * nx, work.data(), sign);
+            for (long task = 0; task < (rows + kBlock - 1) / kBlock; ++task) {
+                const long r0 = task * kBlock;
+                const int bw = static_cast<int>(std::min<long>(kBlock, rows - r0));
+                Cx<Real>* base = a + static_cast<std::size_t>(r0) * nx;
+                for (int b = 0; b < bw; ++b)
+                    for (int x = 0; x < nx; ++x) buf[static_cast<std::size_t>(x) * kBlock + b] = base[static_cast<std::size_t>(b) * nx + x];
+                fx_.exec_batch(buf.data(), work.data(), kBlock, bw, sign);
+                for (int b = 0; b < bw; ++b)
+                    for (int x = 0; x < nx; ++x) base[static_cast<std::size_t>(b) * nx + x] = buf[static_cast<std::size_t>(x) * kBlock + b];
             }
         }
-        // y lines: stride nx; gathered in blocks of adjacent x.
-        constexpr int kBlock = 16;
+        // y lines: stride nx; a block is kBlock adjacent x of one plane
 #pragma omp parallel
         {
-            std::vector<Cx<Real>> buf(static_cast<std::size_t>(ny) * kBlock), work(ny);
+            std::vector<Cx<Real>> buf(static_cast<std::size_t>(ny) * kBlock), work(buf.size());
 #pragma omp for schedule(static)
             for (long task = 0; task < static_cast<long>(nz) * ((nx + kBlock - 1) / kBlock); ++task) {
                 const int z = static_cast<int>(task / ((nx + kBlock - 1) / kBlock));
@@ -211,26 +253,28 @@ public:
                 const int bw = std::min(kBlock, nx - x0);
                 Cx<Real>* plane = a + static_cast<std::size_t>(z) * nx * ny;
                 for (int y = 0; y < ny; ++y)
-                    for (int b = 0; b < bw; ++b) buf[b * ny + y] = plane[static_cast<std::size_t>(y) * nx + x0 + b];
-                for (int b = 0; b < bw; ++b) fy_.exec(buf.data() + b * ny, work.data(), sign);
+                    for (int b = 0; b < bw; ++b)
+                        buf[static_cast<std::size_t>(y) * kBlock + b] = plane[static_cast<std::size_t>(y) * nx + x0 + b];
+                fy_.exec_batch(buf.data(), work.data(), kBlock, bw, sign);
                 for (int y = 0; y < ny; ++y)
-                    for (int b = 0; b < bw; ++b) plane[static_cast<std::size_t>(y) * nx + x0 + b] = buf[b * ny + y];
+                    for (int b = 0; b < bw; ++b)
+                        plane[static_cast<std::size_t>(y) * nx + x0 + b] = buf[static_cast<std::size_t>(y) * kBlock + b];
             }
         }
-        // z lines: stride nx*ny.
+        // z lines: stride nx*ny; a block is kBlock adjacent points of a plane
         const std::size_t plane = static_cast<std::size_t>(nx) * ny;
 #pragma omp parallel
         {
-            std::vector<Cx<Real>> buf(static_cast<std::size_t>(nz) * kBlock), work(nz);
+            std::vector<Cx<Real>> buf(static_cast<std::size_t>(nz) * kBlock), work(buf.size());
 #pragma omp for schedule(static)
             for (long task = 0; task < static_cast<long>((plane + kBlock - 1) / kBlock); ++task) {
                 const std::size_t p0 = static_cast<std::size_t>(task) * kBlock;
                 const int bw = static_cast<int>(std::min<std::size_t>(kBlock, plane - p0));
                 for (int z = 0; z < nz; ++z)
-                    for (int b = 0; b < bw; ++b) buf[b * nz + z] = a[z * plane + p0 + b];
-                for (int b = 0; b < bw; ++b) fz_.exec(buf.data() + b * nz, work.data(), sign);
+                    for (int b = 0; b < bw; ++b) buf[static_cast<std::size_t>(z) * kBlock + b] = a[z * plane + p0 + b];
+                fz_.exec_batch(buf.data(), work.data(), kBlock, bw, sign);
                 for (int z = 0; z < nz; ++z)
-                    for (int b = 0; b < bw; ++b) a[z * plane + p0 + b] = buf[b * nz + z];
+                    for (int b = 0; b < bw; ++b) a[z * plane + p0 + b] = buf[static_cast<std::size_t>(z) * kBlock + b];
             }
         }
     }

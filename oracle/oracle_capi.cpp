@@ -3,8 +3,10 @@
 // cpu_baseline leg. Fields cross the boundary as interleaved doubles in the
 // reference's C2C layout (id = ix + n*(iy + n*iz), x fastest).
 #include "oracle.hpp"
+#include "legacy.hpp"
 
 #include <cstring>
+#include <omp.h>
 #include <memory>
 #include <string>
 
@@ -212,6 +214,56 @@ int orc_run_taylor_green(int n, int steps, int max_steps, int output_every, int 
         fill_row(row, final_row10);
         fill_counters(c, counters6);
         *steps_taken = st;
+    });
+}
+
+// Legacy solver restatement (legacy.hpp): physical velocity phys_in
+// (3 * nx*ny*nz doubles, x fastest; NULL = taylor_green(1, 1)) -> to_spectral,
+// project, truncate; then `steps` RK3 steps of dt. Writes the diagnostics rows
+// (t, E, dissipation, enstrophy, max divergence) after 0..steps steps to
+// diag_out ((steps+1)*5 doubles), the nonlinear term of the initial state to
+// rhs0_out and the final state to state_out (each 3 * 2*nx*ny*nz doubles, C2C
+// interleaved; either may be NULL).
+int orc_legacy_run(const int shape[3], const double lengths[3], double nu, int dealias, int steps, double dt,
+                   const double* phys_in, double* diag_out, double* rhs0_out, double* state_out) {
+    return guarded([&] {
+        oracle::LegacyNS s({shape[0], shape[1], shape[2]}, {lengths[0], lengths[1], lengths[2]}, nu, dealias != 0);
+        const std::size_t np = s.physical_size(), ns = s.spectral_storage_size();
+        oracle::LegacyNS::Vec3 u;
+        if (phys_in) {
+            for (int c = 0; c < 3; ++c) u[c].assign(phys_in + c * np, phys_in + (c + 1) * np);
+        } else {
+            u = s.taylor_green();
+        }
+        auto u_hat = s.to_spectral(u);
+        s.project(u_hat);
+        s.truncate(u_hat);
+        if (rhs0_out) {
+            const auto r = s.nonlinear_rhs(u_hat);
+            for (int c = 0; c < 3; ++c) std::memcpy(rhs0_out + c * ns, r[c].data(), ns * sizeof(double));
+        }
+        double t = 0.0;
+        for (int k = 0; k <= steps; ++k) {
+            if (k > 0) {
+                u_hat = s.step_rk3(u_hat, dt);
+                t += dt;
+            }
+            if (diag_out) {
+                const auto row = s.diagnostics(u_hat, t);
+                const double v[5] = {row.t, row.energy, row.dissipation, row.enstrophy, row.max_spectral_divergence};
+                std::memcpy(diag_out + 5 * k, v, sizeof(v));
+            }
+        }
+        if (state_out)
+            for (int c = 0; c < 3; ++c) std::memcpy(state_out + c * ns, u_hat[c].data(), ns * sizeof(double));
+    });
+}
+
+// Threads of the oracle's OpenMP loops (1 = the reference's serial build).
+int orc_set_threads(int threads) {
+    return guarded([&] {
+        if (threads < 1) throw std::runtime_error("threads must be positive");
+        omp_set_num_threads(threads);
     });
 }
 

@@ -86,6 +86,8 @@ SYMBOLS = [
     ("sph_solver_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sph_solver_synchronize", ctypes.c_int, [_VP]),
     ("sph_solver_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
+    ("sph_solver_time_step_passes", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, _DP, _DP]),
+    ("sph_solver_step_adaptive", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_double, _DP, _DP, _IP]),
     ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _IP, _DP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
@@ -358,6 +360,27 @@ class TaylorGreenSolver:
         by = np.zeros(len(_PASS_NAMES))
         _check(load_library().sph_solver_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
+
+    def time_step_passes(self, dt: float, steps: int = 3):
+        """Pass kernels timed inside `steps` real steps (direct launches, HIP
+        events between them; the state advances): {pass: (mean ms, bytes)}
+        for the four in-step passes, plus "inverse_z" (0 unless the velocity
+        pairs had to be formed first) and "step_ms"."""
+        ms = np.zeros(len(_PASS_NAMES) + 1)
+        by = np.zeros(len(_PASS_NAMES))
+        _check(load_library().sph_solver_time_step_passes(self._h, float(dt), int(steps), _dptr(ms), _dptr(by)))
+        out = {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
+        out["step_ms"] = float(ms[-1])
+        return out
+
+    def step_adaptive(self, dt_max: float, cfl: float):
+        """One adaptive step (dt from the CFL condition): returns
+        (max |u| before the step, dt taken, whether max |u| came from the
+        step's own x pass)."""
+        v, d, f = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(load_library().sph_solver_step_adaptive(self._h, float(dt_max), float(cfl), ctypes.byref(v),
+                                                       ctypes.byref(d), ctypes.byref(f)))
+        return v.value, d.value, bool(f.value)
 
     def tuning(self) -> dict:
         """Autotuner choice per tuned pass: (variant, lines per tile, ms of

@@ -104,14 +104,18 @@ private:
     hipStream_t s_ = nullptr;
 };
 
-// e^{-2 pi i m / n}, m in [0, n), from long double and rounded once.
+// e^{-2 pi i m / n} from long double, rounded once.
+template <typename T>
+cx<T> twiddle_entry(int m, int n) {
+    const long double ang = -2.0L * static_cast<long double>(kPi) * m / n;
+    return {static_cast<T>(std::cos(ang)), static_cast<T>(std::sin(ang))};
+}
+
+// twiddle_entry(m, n) for m in [0, n)
 template <typename T>
 std::vector<cx<T>> twiddle_table(int n) {
     std::vector<cx<T>> tw(n);
-    for (int m = 0; m < n; ++m) {
-        const long double ang = -2.0L * static_cast<long double>(kPi) * m / n;
-        tw[m] = {static_cast<T>(std::cos(ang)), static_cast<T>(std::sin(ang))};
-    }
+    for (int m = 0; m < n; ++m) tw[m] = twiddle_entry<T>(m, n);
     return tw;
 }
 

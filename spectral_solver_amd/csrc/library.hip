@@ -414,6 +414,28 @@ int sph_solver_time_passes(sph_solver_t solver, int reps, double* ms, double* by
     });
 }
 
+int sph_solver_time_step_passes(sph_solver_t solver, double dt, int steps, double* ms, double* bytes) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(ms);
+        require_pointer(bytes);
+        if (!(dt > 0.0)) throw std::runtime_error("dt must be positive");
+        solver->impl->time_step_passes(dt, steps, ms, bytes);
+    });
+}
+
+int sph_solver_step_adaptive(sph_solver_t solver, double dt_max, double cfl, double* max_velocity, double* dt,
+                             int* fused) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(max_velocity);
+        require_pointer(dt);
+        if (!(dt_max > 0.0) || !(cfl > 0.0)) throw std::runtime_error("--dt, --cfl, and --output-dt must be positive");
+        solver->impl->step_adaptive(dt_max, cfl, max_velocity, dt);
+        if (fused) *fused = solver->impl->fused_velocity_max() ? 1 : 0;
+    });
+}
+
 int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms) {
     return guarded([&] {
         require_handle(solver);

@@ -504,14 +504,19 @@ public:
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         hip_check(hipGetLastError(), "time_passes");
-        // algorithmic bytes (each array read or written once), C = one
-        // complex half-spectrum field; the last pass reads 12C + u, g and
-        // writes u, g and the next stage's 6C of velocity pairs (its state
-        // read-back comes from L2)
+        pass_bytes(bytes);
+        hip_check(hipStreamSynchronize(s), "time_passes");
+    }
+
+    // Algorithmic bytes per launch of each pass (each array read or written
+    // once), C = one complex half-spectrum field of every member: inverse z
+    // 3C + 6C; inverse y 6C + 6C; x pass 6C + 12C; forward y 12C + 12C;
+    // forward z + update + next inverse z 12C + u, g (6C) in, u, g (6C) and
+    // the next stage's velocity pairs (6C) out (its state comes from LDS).
+    void pass_bytes(double* bytes) const {
         const double C = static_cast<double>(modes) * 2.0 * sizeof(T) * count;
         const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 30 * C};
         for (int k = 0; k < kPasses; ++k) bytes[k] = b5[k];
-        hip_check(hipStreamSynchronize(s), "time_passes");
     }
 
 private:
@@ -690,6 +695,9 @@ public:
     virtual void set_graph(bool on) = 0;
     virtual void synchronize() = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
+    virtual void time_step_passes(double dt, int steps, double* ms, double* bytes) = 0;
+    virtual void step_adaptive(double dt_max, double cfl, double* vmax, double* dt_used) = 0;
+    virtual bool fused_velocity_max() const = 0;
     virtual void tuning(int* variant, int* lines, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
@@ -805,7 +813,72 @@ public:
     void set_graph(bool on) override { use_graph_ = on; }
     void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }
     void time_passes(int reps, double* ms, double* bytes) override { g_.time_passes(reps, ms, bytes, stream_.get()); }
+
+    // `steps` real RK3 steps of dt with the pass kernels launched directly
+    // (no graph) and a HIP event between consecutive launches on the solver's
+    // stream: ms[k] = mean duration of pass k inside the steps (k = 1..4, over
+    // steps x 3 stages; ms[0] = the standalone inverse z if the velocity
+    // pairs had to be formed first, else 0), ms[5] = mean step time, bytes[k]
+    // = algorithmic bytes per launch (Group::time_passes). The state advances.
+    void time_step_passes(double dt, int steps, double* ms, double* bytes) override {
+        const hipStream_t s = stream_.get();
+        if (steps < 1) steps = 1;
+        g_.set_dt(&dt, s);
+        constexpr int K = Group<T>::kPasses;
+        const int per_step = 3 * (K - 1);
+        std::vector<hipEvent_t> ev(static_cast<std::size_t>(steps) * per_step + 3);
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        std::size_t q = 0;
+        hip_check(hipEventRecord(ev[q++], s), "hipEventRecord");
+        const bool prep = !g_.pairs_valid();
+        g_.prepare_pairs(s);
+        hip_check(hipEventRecord(ev[q++], s), "hipEventRecord");
+        for (int k = 0; k < steps; ++k)
+            for (int st = 0; st < 3; ++st)
+                for (int p = 1; p < K; ++p) {
+                    g_.launch_pass(p, st, g_.U(), g_.G(), 0, 1, s);
+                    hip_check(hipEventRecord(ev[q++], s), "hipEventRecord");
+                }
+        hip_check(hipGetLastError(), "time_step_passes");
+        hip_check(hipEventSynchronize(ev[q - 1]), "hipEventSynchronize");
+        auto span = [&](std::size_t a, std::size_t b) {
+            float e = 0.f;
+            hip_check(hipEventElapsedTime(&e, ev[a], ev[b]), "hipEventElapsedTime");
+            return static_cast<double>(e);
+        };
+        for (int p = 0; p <= K; ++p) ms[p] = 0.0;
+        ms[0] = prep ? span(0, 1) : 0.0;
+        std::size_t e = 1;
+        for (int k = 0; k < steps; ++k)
+            for (int st = 0; st < 3; ++st)
+                for (int p = 1; p < K; ++p, ++e) ms[p] += span(e, e + 1) / (3.0 * steps);
+        ms[K] = span(1, e) / steps;
+        for (auto& x : ev) (void)hipEventDestroy(x);
+        g_.pass_bytes(bytes);
+        prof_.step_calls += steps;
+        prof_.rhs_calls += 3 * steps;
+        prof_.forward_calls += 36 * steps;
+        prof_.backward_calls += 18 * steps;
+    }
+
     void tuning(int* variant, int* lines, double* ms) const override { g_.tuning(variant, lines, ms); }
+
+    // One adaptive step as the driver takes it (impl.hpp:1363-1378, without
+    // the output-time clipping): dt = min(dt_max, cfl * (2 pi / n) / max|u|),
+    // max|u| from the step's own x pass where that form runs (step_cfl).
+    void step_adaptive(double dt_max, double cfl, double* vmax, double* dt_used) override {
+        const double h = 2.0 * kPi / static_cast<double>(n);
+        double v = 0.0;
+        const double dt = step_cfl([&](double vm) {
+            v = vm;
+            double d = dt_max;
+            if (vm > 1.0e-14) d = std::min(d, cfl * h / vm);
+            return d;
+        });
+        *vmax = v;
+        *dt_used = dt;
+    }
+    bool fused_velocity_max() const override { return g_.x_reduces_velocity(); }
 
 private:
     Stream stream_;
