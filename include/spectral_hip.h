@@ -102,6 +102,22 @@ int sph_fft_spectral_modes(sph_fft_t fft, int* mx, int* my, int* mz);
 int sph_fft_forward(sph_fft_t fft, const void* physical, void* spectral, void* stream);
 int sph_fft_backward(sph_fft_t fft, const void* spectral, void* physical, void* stream);
 
+/* Backward transform that allocates nothing. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The same
+ * kernels and results as sph_fft_backward of a handle of this shape and
+ * precision, without a handle: the intermediate half-spectrum field and the
+ * twiddle tables live in a static device array of the library, which every
+ * GPU it runs on holds (80 MiB + 64 KiB) until the process ends. The
+ * intermediate needs (shape[0]/2+1)*shape[1]*shape[2] complex values of the
+ * precision; above 80 MiB the call fails ("... bytes of scratch ...") before
+ * launching anything. Every call on one GPU shares that array and leaves its
+ * contents behind (the twiddles are reloaded only when the shape or precision
+ * changes), so calls that overlap on one GPU race: issue them on one stream,
+ * or order them. stream: a hipStream_t (NULL = the default stream). */
+int sph_fft_backward_noalloc(const int shape[3], int precision, const void* spectral, void* physical,
+                             void* stream);
+
 /* Host-vector transforms with the reference's std::vector<double> contract
  * (fftw_fft.cpp:53-85): physical is physical_size doubles; spectral is
  * 2*spectral_modes interleaved doubles. Synchronous. */

@@ -71,6 +71,7 @@ SYMBOLS = [
     ("sph_fft_spectral_modes", ctypes.c_int, [_VP, _IP, _IP, _IP]),
     ("sph_fft_forward", ctypes.c_int, [_VP, _VP, _VP, _VP]),
     ("sph_fft_backward", ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    ("sph_fft_backward_noalloc", ctypes.c_int, [_IP, ctypes.c_int, _VP, _VP, _VP]),
     ("sph_fft_forward_host", ctypes.c_int, [_VP, _DP, _DP]),
     ("sph_fft_backward_host", ctypes.c_int, [_VP, _DP, _DP]),
     ("sph_solver_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_VP)]),
