@@ -32,6 +32,7 @@ HBM_PEAK_GBS = 8000.0
 PUBLISHED_MPTS_512_FP32 = 512 ** 3 / 0.5694 / 1e6  # 569.4 ms per RK3 step
 PASS_NAMES = ("inverse_z", "inverse_y", "x_products", "forward_y", "forward_z_update")
 KERNEL_NAMES = ("k_pinv_z", "k_plines_inv", "k_px_products_rd", "k_plines_fwd", "k_pfwd_z_rk_inv")
+KERNEL_NAMES_TRANSPOSED = ("k_pinv_z", "k_plines_t_inv", "k_px_products_rd_xt", "k_plines_t_fwd", "k_pfwd_z_rk_inv")
 ENSEMBLE_SIZES = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
 
 
@@ -304,6 +305,9 @@ def main():
     ap.add_argument("--instep-steps", type=int, default=3,
                     help="real steps whose pass kernels are timed with events for the roofline")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-isolated", action="store_true",
+                    help="skip the isolated relaunch timing of the passes (profiled runs: kernel averages then "
+                         "cover in-step launches only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-fft-compare", action="store_true",
@@ -354,10 +358,14 @@ def main():
     # isolation (sph_solver_time_passes) for comparison
     instep = s32.time_step_passes(args.dt, args.instep_steps)
     log("in-step pass ms:", {k: round(v[0], 4) if isinstance(v, tuple) else round(v, 4) for k, v in instep.items()})
-    passes = s32.time_passes(args.reps)
-    log("isolated pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
+    passes = None
+    if not args.no_isolated:
+        passes = s32.time_passes(args.reps)
+        log("isolated pass ms:", {k: round(v[0], 4) for k, v in passes.items()})
     tuning = s32.tuning()
     log("autotuner (variant, lines per tile, ms per variant):", tuning)
+    layout = s32.layout()
+    log("work layout (transposed kept, ms of passes 1-3 line / transposed):", layout)
 
     fp64 = None
     eps_err = None
@@ -376,26 +384,29 @@ def main():
 
     in_step = [k for k in range(1, len(PASS_NAMES))]  # the standalone inverse z runs only after set-up
     dom = max(in_step, key=lambda k: instep[PASS_NAMES[k]][0])
-    kernel = KERNEL_NAMES[dom]
-    if PASS_NAMES[dom] == "x_products" and tuning["x_products"][0] == 2:
+    kernel = (KERNEL_NAMES_TRANSPOSED if layout[0] else KERNEL_NAMES)[dom]
+    if PASS_NAMES[dom] == "x_products" and tuning["x_products"][0] == 2 and not layout[0]:
         kernel = "k_px_products"  # the LDS-staged form
     ms, by = instep[PASS_NAMES[dom]]
-    iso_ms = passes[PASS_NAMES[dom]][0]
+    iso_ms = passes[PASS_NAMES[dom]][0] if passes else None
     achieved = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel, n),
                 "kernel": f"{PASS_NAMES[dom]} ({kernel})", "launch_ms": round(ms, 4),
                 "launch_ms_source": f"mean over the {3 * args.instep_steps} stages of {args.instep_steps} real steps, "
                                     "HIP events between direct launches on the solver's stream",
-                "isolated_launch_ms": round(iso_ms, 4),
-                "isolated_frac": round(by / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "isolated_launch_ms": round(iso_ms, 4) if iso_ms else None,
+                "isolated_frac": round(by / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if iso_ms else None,
                 "algorithmic_bytes": by,
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "step_frac_implemented_bytes": round(implemented_step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "passes_in_step_ms": {k: round(instep[k][0], 4) for k in PASS_NAMES[1:]},
                 "passes_in_step_gbs": {k: round(instep[k][1] / (instep[k][0] * 1e-3) / 1e9, 1) for k in PASS_NAMES[1:]},
-                "passes_isolated_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()},
-                "autotuned": {k: {"variant": v[0], "lines": v[1]} for k, v in tuning.items()}}
+                "passes_isolated_gbs": ({k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()}
+                                        if passes else None),
+                "autotuned": {k: {"variant": v[0], "lines": v[1]} for k, v in tuning.items()},
+                "work_layout": {"transposed": layout[0], "passes_1_3_ms_line": round(layout[1][0], 4),
+                                "passes_1_3_ms_transposed": round(layout[1][1], 4)}}
     s32.close()
 
     copy_gbs = sph.copy_bandwidth(2 << 30, 10)

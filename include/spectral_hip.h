@@ -209,6 +209,17 @@ int sph_solver_step_adaptive(sph_solver_t solver, double dt_max, double cfl, dou
  * 2 = LDS-staged default width. */
 int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms);
 
+/* The work layout the autotuner kept. Between the inverse-y pass, the x pass
+ * and the forward-y pass the pair fields are either in the line layout
+ * [kx][z][y] (an x tile reads n/2+1 rows n*n pairs apart) or, at lengths
+ * >= 256 where a second work buffer of 12 half-spectrum fields fits, in the
+ * transposed layout [z][y][kx] (an x tile is one contiguous block; the y
+ * passes transpose on the way). *transposed = 1 for the latter;
+ * ms[0], ms[1] = summed launch time of those three passes with each layout's
+ * fastest tile variants (0 where a layout was not timed). SPH_LAYOUT=0 / 1
+ * in the environment forces a layout. */
+int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms);
+
 /* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,
  * timed with HIP events over `reps` launches; *gbs = 2*bytes/time (read +
  * write) in GB/s. The measured ceiling against which the pass kernels are

@@ -90,6 +90,7 @@ SYMBOLS = [
     ("sph_solver_time_step_passes", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, _DP, _DP]),
     ("sph_solver_step_adaptive", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_double, _DP, _DP, _IP]),
     ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _IP, _DP]),
+    ("sph_solver_layout", ctypes.c_int, [_VP, _IP, _DP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
     ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
@@ -382,6 +383,13 @@ class TaylorGreenSolver:
         _check(load_library().sph_solver_step_adaptive(self._h, float(dt_max), float(cfl), ctypes.byref(v),
                                                        ctypes.byref(d), ctypes.byref(f)))
         return v.value, d.value, bool(f.value)
+
+    def layout(self):
+        """(transposed work layout kept, [ms of passes 1-3 in the line layout,
+        in the transposed layout])."""
+        t, ms = ctypes.c_int(), np.zeros(2)
+        _check(load_library().sph_solver_layout(self._h, ctypes.byref(t), _dptr(ms)))
+        return bool(t.value), [float(x) for x in ms]
 
     def tuning(self) -> dict:
         """Autotuner choice per tuned pass: (variant, lines per tile, ms of

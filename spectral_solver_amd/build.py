@@ -88,8 +88,12 @@ def build(verbose: bool = True) -> str:
     mpi = find_mpi()
     if mpi:
         inc, lib = mpi
+        # the MPI library directory may hold an older libstdc++ than the HIP
+        # runtime needs: the compiler's own libstdc++ directory comes first
+        stdcxx = subprocess.run([CXX, "-print-file-name=libstdc++.so.6"], capture_output=True, text=True).stdout
+        first = ["-Wl,-rpath," + os.path.dirname(stdcxx.strip())] if os.path.isabs(stdcxx.strip()) else []
         _run([[*host[:4], "-I" + inc, os.path.join(CSRC, "mpi_comm_check.cpp"), "-o", MPI_CHECK, *host[4:],
-               "-L" + lib, "-Wl,-rpath," + lib, "-lmpi"]])
+               *first, "-L" + lib, "-Wl,-rpath," + lib, "-lmpi"]])
     elif os.path.exists(MPI_CHECK):
         os.remove(MPI_CHECK)
     if verbose:

@@ -446,6 +446,15 @@ int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms)
     });
 }
 
+int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(transposed);
+        require_pointer(ms);
+        solver->impl->layout(transposed, ms);
+    });
+}
+
 int sph_measure_copy_bandwidth(int64_t bytes, int reps, double* gbs) {
     return guarded([&] {
         require_pointer(gbs);

@@ -294,6 +294,79 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
 }
 
 // ---------------------------------------------------------------------------
+// y lines between the line layout and the transposed (kx-fastest) layout of
+// the x pass:
+//   line layout        [kx][z][y][2]   offset ((kx*n + z)*n + y)*2 + f
+//   transposed layout  [z][y][kx][2]   offset ((z*n + y)*H + kx)*2 + f
+// (per pair field; fields `field_stride` complex values apart, members
+// `member_stride` apart; grid (kx blocks of NL/2, z, field + nfields*member)).
+// In the transposed layout an x tile (NL y lines of one z, every kx) is one
+// contiguous block instead of H rows n*n pairs apart, which at n = 1024 cost
+// the x pass its address translations. A tile here is NL/2 pair lines of
+// consecutive kx at one z: on the line-layout side each pair line is 2n
+// contiguous values, on the transposed side each y row holds the tile's NL
+// values contiguously. INV: line layout (spectral ky) in, transposed
+// (physical y) out; forward: transposed in, line layout out. Out of place.
+// ---------------------------------------------------------------------------
+template <typename T, int N, bool INV, int LINES>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
+    k_plines_t(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+               const cx<T>* tw) {
+    using TL = Tile<T, N, LINES>;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E;
+    constexpr int H = N / 2 + 1;
+    static_assert(NL % 2 == 0, "pair tiles need an even line count");
+    constexpr bool SWZ = sizeof(T) == 4 && NL <= 8;
+    __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
+    const int kx0 = tile.x * (NL / 2);
+    const int z = tile.y;
+    const int field = tile.z % nfields, member = tile.z / nfields;
+    const long long fo = field * field_stride + member * member_stride;
+    const int lines = (H - kx0 < NL / 2 ? H - kx0 : NL / 2) * 2;  // valid values of a transposed row
+    const cx<T>* src = in + fo;
+    cx<T>* dst = out + fo;
+    // line layout: value rem of pair line kx0 + g; transposed layout: value
+    // c of the tile's row y
+    auto lin_off = [&](int g, int rem) -> long long {
+        return ((static_cast<long long>(kx0 + g) * N + z) * N) * 2 + rem;
+    };
+    const long long row0 = (static_cast<long long>(z) * N * H + kx0) * 2;
+    auto tr_off = [&](int y, int c) -> long long { return row0 + static_cast<long long>(y) * H * 2 + c; };
+    load_twiddles<T, N, NT>(twl, tw, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int m = tid + i * NT;
+        if (TL::ELEMS % NT == 0 || m < TL::ELEMS) {
+            if constexpr (INV) {
+                const int g = m / (2 * N), rem = m % (2 * N);  // pair line g, element rem >> 1, member rem & 1
+                buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = 2 * g < lines ? src[lin_off(g, rem)] : czero<T>();
+            } else {
+                const int y = m / NL, c = m % NL;  // row y, value c = 2g + f
+                buf[tile_at<NL, SWZ>(y, c)] = c < lines ? src[tr_off(y, c)] : czero<T>();
+            }
+        }
+    }
+    __syncthreads();
+    fft_tile<T, N, NL, NT, INV, 0, 1, SWZ>(buf, twl, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int m = tid + i * NT;
+        if (TL::ELEMS % NT == 0 || m < TL::ELEMS) {
+            if constexpr (INV) {
+                const int y = m / NL, c = m % NL;
+                if (c < lines) dst[tr_off(y, c)] = buf[tile_at<NL, SWZ>(y, c)];
+            } else {
+                const int g = m / (2 * N), rem = m % (2 * N);
+                if (2 * g < lines) dst[lin_off(g, rem)] = buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // x-pass building blocks. A tile is NL x lines (y0 .. y0+NL-1, fixed z) in LDS,
 // element x of line l at buf[x*S + l]; thread tid handles element
 // idx = tid + i*NT (x = idx % N, l = idx / N) of the physical lines.
@@ -455,7 +528,10 @@ constexpr int kVmaxStride = 16;
 // agree to rounding; the compiler may contract multiply-adds differently).
 // VMAX: also reduce max |u|^2 of the physical velocity into a.vmax2 (the
 // adaptive time step's CFL condition, without transforms of its own).
-template <typename T, int N, int LINES = 0, bool VMAX = false>
+// XT: the transposed layout [z][y][kx][2] in and out (k_plines_t): the tile's
+// NL lines are one contiguous block of NL * (N/2+1) pairs, row k of line l at
+// l * (N/2+1) + k.
+template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 // The second bound is the minimum number of waves per SIMD: 512-thread tiles
 // ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
 // at 128 like the 1024-thread form.
@@ -497,12 +573,18 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     // less than 2 GiB (everything but FP64 at N = 1024); the launcher only
     // runs this form with ny * nz <= N * N
     constexpr bool kNarrow = static_cast<long long>(Z::STEP) * N * N * sizeof(cx2<T>) < (1LL << 31);
+    constexpr uint32_t H = N / 2 + 1;
+    // transposed layout: the tile's lines as one block (line l at l * H)
+    const size_t xt_block = (static_cast<size_t>(z) * a.ny + y0) * H;
     load_twiddles<T, N, NT>(twl, a.tw, tid);
     // u, s at slot m of element j + m*N/E (j = tid / NL)
     T u[3][E], s[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + static_cast<size_t>(z) * a.ny + y0;
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + (XT ? xt_block : static_cast<size_t>(z) * a.ny + y0);
+        // XT: one resource for the tile's block; lanes past the last line read zeros
+        const __amdgpu_buffer_rsrc_t rsx = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<cx2<T>*>(AB), 0, static_cast<int>(NL * H * sizeof(cx2<T>)), 0x00020000);
         cx<T> v[1][T0][R0], w[1][TLS][RL];
 #pragma unroll
         for (int i = 0; i < T0; ++i)
@@ -519,7 +601,10 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 const int e = MIR ? Z::e_first_mirror(tid, r) : Z::e_first(tid, i, r);
                 const int k = e <= N / 2 ? e : N - e;
                 cx2<T> q;
-                if constexpr (kNarrow) {
+                if constexpr (XT) {
+                    const uint32_t off = ok ? (l * H + k) * static_cast<uint32_t>(sizeof(cx2<T>)) : 0xFFFFFFC0u;
+                    q = ld_cx2<T>(rsx, off);
+                } else if constexpr (kNarrow) {
                     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                         const_cast<cx2<T>*>(AB + static_cast<size_t>(kmin) * row), 0,
                         static_cast<int>(STEP * row * sizeof(cx2<T>)), 0x00020000);
@@ -581,7 +666,9 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
             constexpr int HR = RL / 2;
             const int j = Z::j_mirror(tid);
             const bool self0 = j == 0, selfh = j == Z::STEP / 2;
-            cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw) + static_cast<size_t>(z) * a.ny + y0 + l;
+            cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw) +
+                        (XT ? xt_block + static_cast<size_t>(l) * H : static_cast<size_t>(z) * a.ny + y0 + l);
+            const size_t ostep = XT ? 1 : row;  // between rows k
 #pragma unroll
             for (int r = 0; r <= HR; ++r) {
                 const cx<T> zk = w[0][0][r % RL];
@@ -598,7 +685,7 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                 cx2<T> q;
                 q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                 q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                if (ok && (r < HR || self0)) o[static_cast<size_t>(j + r * Z::STEP) * row] = q;
+                if (ok && (r < HR || self0)) o[static_cast<size_t>(j + r * Z::STEP) * ostep] = q;
             }
             continue;
         }
@@ -620,7 +707,10 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
                     cx2<T> q;
                     q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                     q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                    o[k * row + static_cast<size_t>(z) * a.ny + y0 + l2] = q;
+                    if constexpr (XT)
+                        o[xt_block + static_cast<size_t>(l2) * H + k] = q;
+                    else
+                        o[k * row + static_cast<size_t>(z) * a.ny + y0 + l2] = q;
                 }
             }
         }

@@ -34,12 +34,22 @@ struct PassTable {
     void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                    const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
-    void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, hipStream_t);
+    // xt: the transposed work layout (register-direct variants 0 and 1 only)
+    void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, bool xt, hipStream_t);
     // the register-direct x pass of one cubic solve that also reduces max
     // |u|^2 into a.vmax2 (variants 0 and 1; px_products_vmax)
-    void (*px_products_vmax)(const PairXArgs<T>&, int variant, hipStream_t);
+    void (*px_products_vmax)(const PairXArgs<T>&, int variant, bool xt, hipStream_t);
     // forward z + update, then the next stage's inverse z (k_pfwd_z_rk_inv)
     void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, hipStream_t);
+    // transposed work layout between the y passes and the x pass (k_plines_t):
+    // supported (xt), its y-pass tile widths (variants 0..2), and the pass
+    // itself (line layout in and transposed out when inverse, else the other
+    // way round; nfields pair fields field_stride apart, members
+    // member_stride apart)
+    bool xt;
+    int xt_lines[kMaxVariants];
+    void (*plines_t)(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                     int members, bool inverse, const cx<T>* tw, int variant, hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -51,7 +61,7 @@ const PassTable<T>* pass_table(int n);
 // register allocation (a 12-byte spill per thread at 512^3, 0.13 GB more
 // HBM writes per launch).
 template <typename T, int N>
-void px_products_vmax(const PairXArgs<T>& a, int variant, hipStream_t s);
+void px_products_vmax(const PairXArgs<T>& a, int variant, bool xt, hipStream_t s);
 
 // The compiled transform lengths.
 #define SPH_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
@@ -137,13 +147,56 @@ struct Launch {
     // still makes a whole number of wavefronts
     static constexpr int X0 = ZT::NL;
     static constexpr int X1 = X0 / 2 >= 2 && (X0 / 2) * N / ZT::E >= 64 ? X0 / 2 : X0;
-    template <int L>
+    template <int L, bool XT = false>
     static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
         const dim3 grid((ny + L - 1) / L, nz, members);
-        k_px_products_rd<T, N, L><<<grid, ZV::NT, 0, s>>>(a);
+        k_px_products_rd<T, N, L, false, XT><<<grid, ZV::NT, 0, s>>>(a);
     }
-    static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, hipStream_t s) {
+    // the transposed work layout (k_plines_t, k_px_products_rd<..., XT>) at
+    // the lengths where the x pass's rows lie far apart
+    static constexpr bool kXT = N >= 256;
+    // its y tiles: the default width, half and twice of it (twice: 128-byte
+    // transposed rows at n = 1024 in FP32, where the default is 8 lines)
+    static constexpr int T0L = TL::NL;
+    static constexpr int T1L = T0L / 2 >= 2 ? T0L / 2 : T0L;
+    static constexpr int T2L =
+        (tile_lds<2 * T0L, sizeof(T) == 4 && 2 * T0L <= 8>(N) + N) * static_cast<int>(sizeof(cx<T>)) <= kLdsBytes
+            ? 2 * T0L
+            : T0L;
+    template <int L>
+    static void plines_t_l(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                           int members, bool inverse, const cx<T>* tw, hipStream_t s) {
+        using TV = Tile<T, N, L>;
+        constexpr int H = N / 2 + 1;
+        const dim3 grid((H + L / 2 - 1) / (L / 2), N, nfields * members);
+        if (inverse)
+            k_plines_t<T, N, true, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+        else
+            k_plines_t<T, N, false, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+    }
+    static void plines_t(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                         int members, bool inverse, const cx<T>* tw, int variant, hipStream_t s) {
+        if constexpr (kXT) {
+            if (variant == 2)
+                plines_t_l<T2L>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+            else if (variant == 1)
+                plines_t_l<T1L>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+            else
+                plines_t_l<T0L>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+        }
+    }
+    static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, bool xt,
+                            hipStream_t s) {
+        if constexpr (kXT) {
+            if (xt) {
+                if (variant == 1)
+                    px_products_rd<X1, true>(a, ny, nz, members, s);
+                else
+                    px_products_rd<X0, true>(a, ny, nz, members, s);
+                return;
+            }
+        }
         // the register-direct form addresses rows of at most N * N modes
         if (static_cast<long long>(ny) * nz > static_cast<long long>(N) * N) variant = 2;
         if (variant == 2) {
@@ -175,7 +228,10 @@ struct Launch {
                                         &pinv_x,
                                         &px_products,
                                         &px_products_vmax<T, N>,
-                                        &pfwd_z_rk};
+                                        &pfwd_z_rk,
+                                        kXT,
+                                        {T0L, T1L, T2L},
+                                        &plines_t};
 };
 
 }  // namespace sph

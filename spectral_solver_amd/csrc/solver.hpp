@@ -187,12 +187,23 @@ public:
     // per tuned pass: the variant kept, its lines per tile, and the mean
     // launch time of every variant (ms[pass * kMaxVariants + variant]; 0 when
     // not timed or not defined)
+    // (in the layout kept: the transposed layout's variants and times when
+    // it was chosen)
     void tuning(int* variant, int* lines, double* ms) const {
         for (int p = 0; p < kTuned; ++p) {
-            variant[p] = var_[p];
-            lines[p] = pt_->tile_lines[p][var_[p]];
-            for (int v = 0; v < kMaxVariants; ++v) ms[p * kMaxVariants + v] = tune_ms_[p][v];
+            const int v = xt_ ? xvar_[p] : var_[p];
+            variant[p] = v;
+            lines[p] = xt_ && p != 1 ? pt_->xt_lines[v] : pt_->tile_lines[p][v];
+            for (int q = 0; q < kMaxVariants; ++q) ms[p * kMaxVariants + q] = xt_ ? xtune_ms_[p][q] : tune_ms_[p][q];
         }
+    }
+    // the work layout kept (1: transposed between the y passes and the x
+    // pass) and the summed time of passes 1-3 with each layout's fastest
+    // variants (0 where a layout was not timed)
+    void layout(int* transposed, double* ms) const {
+        *transposed = xt_ ? 1 : 0;
+        ms[0] = layout_ms_[0];
+        ms[1] = layout_ms_[1];
     }
 
     // pass kernels: 0 (inverse z) runs only in prepare_pairs, 1-4 per stage
@@ -247,31 +258,43 @@ public:
         cx<T>* W = W_.get() + first * sw;
         u += first * su;
         g += first * su;
+        // transposed layout: the inverse y pass moves the velocity pairs from
+        // W (line layout) to X (transposed), the x pass works in X, the
+        // forward y pass moves the product pairs back to W
+        cx<T>* X = xt_ ? X_.get() + first * sw : nullptr;
         switch (k) {
             case 0:
                 pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, s);
                 break;
             case 1:
-                pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
+                if (xt_)
+                    pt_->plines_t(W, X, 3, M2, sw, members, true, tw_.get(), xvar_[0], s);
+                else
+                    pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
             case 2: {
+                cx<T>* src = xt_ ? X : W;
                 PairXArgs<T> a{};
-                for (int c = 0; c < 3; ++c) a.ab[c] = W + c * M2;
-                for (int q = 0; q < 6; ++q) a.out[q] = W + q * M2;
+                for (int c = 0; c < 3; ++c) a.ab[c] = src + c * M2;
+                for (int q = 0; q < 6; ++q) a.out[q] = src + q * M2;
                 a.ny = n;
                 a.nz = n;
                 a.scale = T(1) / static_cast<T>(npts);
                 a.tw = tw_.get();
                 a.ms_w = sw;
                 a.vmax2 = vmax2;
+                const int xv = xt_ ? xvar_[1] : var_[1];
                 if (vmax2)
-                    pt_->px_products_vmax(a, var_[1], s);
+                    pt_->px_products_vmax(a, xv, xt_, s);
                 else
-                    pt_->px_products(a, n, n, members, var_[1], s);
+                    pt_->px_products(a, n, n, members, xv, xt_, s);
                 break;
             }
             case 3:
-                pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
+                if (xt_)
+                    pt_->plines_t(X, W, 6, M2, sw, members, false, tw_.get(), xvar_[2], s);
+                else
+                    pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
                 break;
             default: {
                 PairFwdZArgs<T> a{};
@@ -316,7 +339,7 @@ public:
     // dt either (launch_step_mid); the tail is the rest of the step, from
     // stage 0's forward z and update on. One solve per group, register-direct
     // x pass only.
-    bool x_reduces_velocity() const { return count == 1 && var_[1] != 2; }
+    bool x_reduces_velocity() const { return count == 1 && (xt_ || var_[1] != 2); }
     void launch_step_head(unsigned long long* vmax2, hipStream_t s) const {
         k_zero_vmax<<<1, kVmaxSlots, 0, s>>>(vmax2);
         launch_pass(1, 0, U_.get(), G_.get(), 0, 1, s);
@@ -529,52 +552,90 @@ private:
     void autotune() {
         using Key = std::tuple<int, int, int>;
         struct Choice {
-            int var[kTuned];
-            double ms[kTuned][kMaxVariants];
+            int var[kTuned], xvar[kTuned];
+            double ms[kTuned][kMaxVariants], xms[kTuned][kMaxVariants], layout_ms[2];
+            bool xt;
         };
+        // SPH_LAYOUT=0 / 1 forces the line / transposed work layout (the
+        // latter where the length supports it)
+        const char* lay = std::getenv("SPH_LAYOUT");
+        const int forced_layout = lay && (lay[0] == '0' || lay[0] == '1') && lay[1] == '\0' ? lay[0] - '0' : -1;
+        const bool xt_allowed = pt_->xt && forced_layout != 0 && transposed_fits();
         const char* force = std::getenv("SPH_TILE_VARIANT");
         if (force && force[0] >= '0' && force[0] < '0' + kMaxVariants && force[1] == '\0') {
-            for (int p = 0; p < kTuned; ++p) var_[p] = std::min(force[0] - '0', pt_->variants[p] - 1);
+            for (int p = 0; p < kTuned; ++p) {
+                var_[p] = std::min(force[0] - '0', pt_->variants[p] - 1);
+                xvar_[p] = std::min(force[0] - '0', p == 1 ? 1 : kMaxVariants - 1);
+            }
+            use_transposed(forced_layout == 1 && xt_allowed);
             return;
         }
         static std::mutex mu;
-        static std::map<Key, Choice> cache;
-        const Key key{static_cast<int>(sizeof(T)), n, count};
+        static std::map<std::tuple<int, int, int, int>, Choice> cache;
+        const auto key = std::make_tuple(static_cast<int>(sizeof(T)), n, count, forced_layout);
         {
             std::lock_guard<std::mutex> lock(mu);
             const auto it = cache.find(key);
             if (it != cache.end()) {
+                const Choice& c = it->second;
                 for (int p = 0; p < kTuned; ++p) {
-                    var_[p] = it->second.var[p];
-                    for (int v = 0; v < kMaxVariants; ++v) tune_ms_[p][v] = it->second.ms[p][v];
+                    var_[p] = c.var[p];
+                    xvar_[p] = c.xvar[p];
+                    for (int v = 0; v < kMaxVariants; ++v) {
+                        tune_ms_[p][v] = c.ms[p][v];
+                        xtune_ms_[p][v] = c.xms[p][v];
+                    }
                 }
+                layout_ms_[0] = c.layout_ms[0];
+                layout_ms_[1] = c.layout_ms[1];
+                use_transposed(c.xt);
                 return;
             }
         }
         const char* env = std::getenv("SPH_AUTOTUNE");
-        if (env && env[0] == '0') return;
+        if (env && env[0] == '0') {
+            use_transposed(forced_layout == 1 && xt_allowed);
+            return;
+        }
         Stream stream;
         const hipStream_t s = stream.get();
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "hipEventCreate");
         hip_check(hipEventCreate(&b), "hipEventCreate");
         const int pass_of[kTuned] = {1, 2, 3};
-        for (int p = 0; p < kTuned; ++p) {
-            int best = 0;
-            for (int v = 0; v < pt_->variants[p]; ++v) {
-                var_[p] = v;
-                launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
-                hip_check(hipEventRecord(a, s), "hipEventRecord");
-                for (int r = 0; r < 3; ++r) launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
-                hip_check(hipEventRecord(b, s), "hipEventRecord");
-                hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-                float e = 0.f;
-                hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
-                tune_ms_[p][v] = e / 3.0;
-                if (tune_ms_[p][v] < tune_ms_[p][best]) best = v;
+        // every variant of the tuned passes in the current layout; returns the
+        // summed time of the fastest
+        auto tune_layout = [&](int (&var)[kTuned], double (&ms)[kTuned][kMaxVariants], const int* nvar) {
+            double total = 0.0;
+            for (int p = 0; p < kTuned; ++p) {
+                int best = 0;
+                for (int v = 0; v < nvar[p]; ++v) {
+                    var[p] = v;
+                    launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
+                    hip_check(hipEventRecord(a, s), "hipEventRecord");
+                    for (int r = 0; r < 3; ++r) launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
+                    hip_check(hipEventRecord(b, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+                    float e = 0.f;
+                    hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
+                    ms[p][v] = e / 3.0;
+                    if (ms[p][v] < ms[p][best]) best = v;
+                }
+                var[p] = best;
+                total += ms[p][best];
             }
-            var_[p] = best;
+            return total;
+        };
+        use_transposed(false);
+        layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants);
+        bool keep_xt = false;
+        if (xt_allowed) {
+            use_transposed(true);
+            const int nx[kTuned] = {kMaxVariants, 2, kMaxVariants};
+            layout_ms_[1] = tune_layout(xvar_, xtune_ms_, nx);
+            keep_xt = forced_layout == 1 || layout_ms_[1] < layout_ms_[0];
         }
+        use_transposed(keep_xt);
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         // the timing launches left transforms of zeros in W: clear it again
@@ -583,10 +644,31 @@ private:
         Choice c{};
         for (int p = 0; p < kTuned; ++p) {
             c.var[p] = var_[p];
-            for (int v = 0; v < kMaxVariants; ++v) c.ms[p][v] = tune_ms_[p][v];
+            c.xvar[p] = xvar_[p];
+            for (int v = 0; v < kMaxVariants; ++v) {
+                c.ms[p][v] = tune_ms_[p][v];
+                c.xms[p][v] = xtune_ms_[p][v];
+            }
         }
+        c.layout_ms[0] = layout_ms_[0];
+        c.layout_ms[1] = layout_ms_[1];
+        c.xt = keep_xt;
         std::lock_guard<std::mutex> lock(mu);
         cache.emplace(key, c);
+    }
+
+    // The transposed layout needs a second work buffer of 12C per member:
+    // taken only with room to spare on the device.
+    bool transposed_fits() const {
+        std::size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+        const std::size_t need = static_cast<std::size_t>(count) * sw * sizeof(cx<T>);
+        return need + (std::size_t(4) << 30) <= free_b;
+    }
+    void use_transposed(bool on) {
+        xt_ = on;
+        if (on && !X_.get()) X_.resize(static_cast<std::size_t>(count) * sw);
+        if (!on) X_.release();
     }
 
     std::vector<cx<T>> download_state(int m, hipStream_t s) const {
@@ -604,8 +686,15 @@ private:
     std::vector<T> nu_;
     std::vector<double> last_dt_;
     bool pairs_valid_ = false;     // W holds the velocity pairs of U (every member)
-    int var_[kTuned] = {0, 0, 0};  // variant of each tuned pass
+    int var_[kTuned] = {0, 0, 0};  // variant of each tuned pass (line layout)
     double tune_ms_[kTuned][kMaxVariants] = {};
+    // transposed work layout (k_plines_t, XT x pass): in use, its second work
+    // buffer, variants and times; layout_ms_: passes 1-3 in each layout
+    bool xt_ = false;
+    DeviceArray<cx<T>> X_;
+    int xvar_[kTuned] = {0, 0, 0};
+    double xtune_ms_[kTuned][kMaxVariants] = {};
+    double layout_ms_[2] = {0.0, 0.0};
 };
 
 // Per-step GPU time from event pairs, resolved lazily.
@@ -699,6 +788,7 @@ public:
     virtual void step_adaptive(double dt_max, double cfl, double* vmax, double* dt_used) = 0;
     virtual bool fused_velocity_max() const = 0;
     virtual void tuning(int* variant, int* lines, double* ms) const = 0;
+    virtual void layout(int* transposed, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
 };
@@ -862,6 +952,7 @@ public:
     }
 
     void tuning(int* variant, int* lines, double* ms) const override { g_.tuning(variant, lines, ms); }
+    void layout(int* transposed, double* ms) const override { g_.layout(transposed, ms); }
 
     // One adaptive step as the driver takes it (impl.hpp:1363-1378, without
     // the output-time clipping): dt = min(dt_max, cfl * (2 pi / n) / max|u|),
