@@ -88,10 +88,13 @@ def ensemble_leg(sph, steps: int, warmup: int):
     sec = (time.perf_counter() - t0) / steps
     pts = sum(n ** 3 for n in sizes)
     alg = sum(step_bytes(n, 4) for n in sizes)
+    mode = ens.launch_mode()
     passes = ens.time_passes(3)
     ens.close()
     return {"workload": "64 solves: 16 each of 32^3, 48^3, 64^3, 96^3, FP32, nu=1e-2, CFL 0.5 dt, "
                         f"{steps} RK3 steps, one hipGraph per step",
+            "one_launch_per_pass": mode[0],
+            "timed_step_ms": {"per_size_chains": round(mode[1][0], 4), "one_launch_per_pass": round(mode[1][1], 4)},
             "mpts_per_s": round(pts / sec / 1e6, 3), "ms_per_step": round(sec * 1e3, 4),
             "points": pts, "algorithmic_bytes_per_step": alg,
             "roofline_frac": round(alg / sec / 1e9 / HBM_PEAK_GBS, 4),

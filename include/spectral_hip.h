@@ -209,6 +209,13 @@ int sph_solver_step_adaptive(sph_solver_t solver, double dt_max, double cfl, dou
  * 2 = LDS-staged default width. */
 int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms);
 
+/* One tuned pass of the autotuner, any of the four: pass 0 inverse y, 1 x,
+ * 2 forward y (as sph_solver_tuning), 3 forward z + update + next inverse z
+ * (variants: 0 = default z-tile width, 1 = half of it; the standalone inverse
+ * z takes the same width). variant, lines per tile, ms[3] = mean launch time
+ * of each variant (0 where the pass has fewer variants). */
+int sph_solver_tuned_pass(sph_solver_t solver, int pass, int* variant, int* lines, double* ms);
+
 /* The work layout the autotuner kept. Between the inverse-y pass, the x pass
  * and the forward-y pass the pair fields are either in the line layout
  * [kx][z][y] (an x tile reads n/2+1 rows n*n pairs apart) or, at lengths
@@ -255,8 +262,9 @@ int sph_run_taylor_green(const sph_run_config* config, sph_diagnostics* final_ro
  * this is that solver batched: `count` members, each with its own grid size
  * n[j] (any compiled length), viscosity nu[j] and seeded random-phase
  * solenoidal initial field with E(k) ~ k^4 exp(-2 (k/4)^2), scaled to kinetic
- * energy `energy`. Members of equal size share every pass launch (one launch
- * per pass per RK stage per distinct size), and a step is one hipGraph.
+ * energy `energy`. Members of equal size share every pass launch, sizes up
+ * to 128 can share one launch per pass (sph_ensemble_launch_mode), and a
+ * step is one hipGraph.
  * Each member's state follows the same RK3 stage as TaylorGreenSolver::step.
  * ------------------------------------------------------------------------ */
 typedef struct sph_ensemble_s* sph_ensemble_t;
@@ -281,6 +289,16 @@ int sph_ensemble_step_seconds(sph_ensemble_t ens, double* seconds, int64_t* step
 
 /* Pass timings as sph_solver_time_passes, summed over the distinct sizes. */
 int sph_ensemble_time_passes(sph_ensemble_t ens, int reps, double* ms, double* bytes);
+
+/* How the ensemble launches a step. Members of the sizes 16, 32, 48, 64, 96
+ * and 128 can run as one launch per pass for all of them (four launches per
+ * RK stage, one-wavefront tiles of every size in each launch); other sizes,
+ * and all sizes when that is slower, run one chain of launches per size on
+ * parallel branches of the step's graph. Chosen at creation by timing both
+ * (SPH_ENSEMBLE_FUSED=0 / 1 forces it). *one_launch_per_pass = 1 when the
+ * former is used; step_ms[0], step_ms[1] = the timed step with per-size
+ * chains and with one launch per pass (0 where not timed). */
+int sph_ensemble_launch_mode(sph_ensemble_t ens, int* one_launch_per_pass, double* step_ms);
 
 /* The ensemble's initial field for one member, on the host: three
  * components of n*n*(n/2+1) interleaved (re, im) doubles, layout as

@@ -91,6 +91,7 @@ SYMBOLS = [
     ("sph_solver_step_adaptive", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_double, _DP, _DP, _IP]),
     ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _IP, _DP]),
     ("sph_solver_layout", ctypes.c_int, [_VP, _IP, _DP]),
+    ("sph_solver_tuned_pass", ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP, _DP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
     ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
@@ -105,6 +106,7 @@ SYMBOLS = [
     ("sph_ensemble_synchronize", ctypes.c_int, [_VP]),
     ("sph_ensemble_step_seconds", ctypes.c_int, [_VP, _DP, _I64P]),
     ("sph_ensemble_time_passes", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP]),
+    ("sph_ensemble_launch_mode", ctypes.c_int, [_VP, _IP, _DP]),
     ("sph_random_solenoidal_field", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _DP, _DP, _DP]),
 ]
 
@@ -400,7 +402,11 @@ class TaylorGreenSolver:
         _check(load_library().sph_solver_tuning(self._h, var.ctypes.data_as(_IP), lines.ctypes.data_as(_IP),
                                                 _dptr(ms)))
         names = ("inverse_y", "x_products", "forward_y")
-        return {k: (int(var[p]), int(lines[p]), [float(x) for x in ms[3 * p:3 * p + 3]]) for p, k in enumerate(names)}
+        out = {k: (int(var[p]), int(lines[p]), [float(x) for x in ms[3 * p:3 * p + 3]]) for p, k in enumerate(names)}
+        v, l, m3 = ctypes.c_int(), ctypes.c_int(), np.zeros(3)
+        _check(load_library().sph_solver_tuned_pass(self._h, 3, ctypes.byref(v), ctypes.byref(l), _dptr(m3)))
+        out["forward_z_update"] = (v.value, l.value, [float(x) for x in m3])
+        return out
 
     def u_hat(self, c: int) -> np.ndarray:
         """Component c as complex [kx in 0..n/2][kz][ky]."""
@@ -522,6 +528,13 @@ class Ensemble:
         by = np.zeros(len(_PASS_NAMES))
         _check(load_library().sph_ensemble_time_passes(self._h, int(reps), _dptr(ms), _dptr(by)))
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
+
+    def launch_mode(self):
+        """(one launch per pass for the fused sizes in use, [step ms with
+        per-size chains, with one launch per pass])."""
+        f, ms = ctypes.c_int(), np.zeros(2)
+        _check(load_library().sph_ensemble_launch_mode(self._h, ctypes.byref(f), _dptr(ms)))
+        return bool(f.value), [float(x) for x in ms]
 
 
 def run_taylor_green(config: RunConfig) -> Diagnostics:

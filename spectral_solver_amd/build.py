@@ -30,7 +30,7 @@ ARCH = "gfx950"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INCLUDE, "-I" + CSRC]
-UNITS = ["passes_f32.hip", "passes_f64.hip", "passes_vmax.hip", "library.hip", "noalloc.hip"]
+UNITS = ["passes_f32.hip", "passes_f64.hip", "passes_f32_b.hip", "passes_f64_b.hip", "passes_vmax.hip", "library.hip", "noalloc.hip", "passes_ens.hip"]
 
 
 def find_mpi():

@@ -86,20 +86,24 @@ constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 
 // elements per field: a radix-8 stage next to a radix-3 one would make that 24,
 // so lengths with a factor 3 take radix-4 stages and, for an odd power of two,
 // one radix-6 stage (2*3) in place of a 2 and a 3: 12 elements per thread and
-// the fewest stages (96 = 4*4*6, 24 = 4*6, 48 = 4*4*3, 12 = 4*3). Powers of
-// two with two or more radix-8 stages put one of them last (1024 = 8*4*4*8,
-// 256 = 8*4*8): the same stages, and the last one then holds one task per
-// thread (ZTile::kMirrorableLast).
+// the fewest stages (96 = 4*4*6, 24 = 4*6, 48 = 4*4*3, 12 = 4*3, 384 =
+// 4*4*4*6). Lengths with a factor 5 likewise take radix-4 stages, a radix-2
+// one for an odd power of two, then the 5s: 20 elements per thread (80 =
+// 4*4*5, 160 = 4*4*2*5, 640 = 4*4*4*2*5) instead of 40 next to a radix 8.
+// Powers of two with two or more radix-8 stages put one of them last (1024 =
+// 8*4*4*8, 256 = 8*4*8): the same stages, and the last one then holds one
+// task per thread (ZTile::kMirrorableLast).
 constexpr RadixPlan z_radix_plan(int n) {
     const RadixPlan p = radix_plan(n);
     if (p.count < 0) return p;
-    bool has3 = false;
+    bool has3 = false, has5 = false;
     int eights = 0;
     for (int s = 0; s < p.count; ++s) {
         has3 |= p.r[s] == 3;
+        has5 |= p.r[s] == 5;
         eights += p.r[s] == 8;
     }
-    if (!has3) {
+    if (!has3 && !has5) {
         RadixPlan q = p;
         if (eights >= 2 && p.r[0] == 8 && p.r[1] == 8 && p.r[p.count - 1] != 8) {
             for (int s = 1; s < p.count - 1; ++s) q.r[s] = p.r[s + 1];
@@ -123,8 +127,12 @@ constexpr RadixPlan z_radix_plan(int n) {
     RadixPlan q{};
     for (int i = 0; i < twos / 2; ++i) q.r[q.count++] = 4;
     if (twos % 2) {
-        q.r[q.count++] = 6;
-        --threes;
+        if (threes > 0) {
+            q.r[q.count++] = 6;
+            --threes;
+        } else {
+            q.r[q.count++] = 2;
+        }
     }
     for (int i = 0; i < threes; ++i) q.r[q.count++] = 3;
     for (int i = 0; i < fives; ++i) q.r[q.count++] = 5;

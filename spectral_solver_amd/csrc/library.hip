@@ -78,7 +78,7 @@ public:
         fx.ny = ny;
         fx.nz = nz;
         fx.tw = tw_[0].get();
-        pt_[0]->fwd_x(fx, FX_PLAIN, ny, nz, s);
+        pt_[0]->fwd_x(fx, ny, nz, s);
         pt_[1]->lines(out, static_cast<long long>(H) * nz, false, tw_[1].get(), s);
         FwdZArgs<T> fz{};
         fz.in[0] = out;
@@ -99,9 +99,8 @@ public:
         iz.out[0] = scratch_.get();
         iz.nfields = 1;
         iz.ny = ny;
-        iz.n = nz;
         iz.tw = tw_[2].get();
-        pt_[2]->inv_z(iz, IZ_PLAIN, ny, H, s);
+        pt_[2]->inv_z(iz, ny, H, s);
         pt_[1]->lines(scratch_.get(), static_cast<long long>(H) * nz, true, tw_[1].get(), s);
         InvXArgs<T> ix{};
         ix.a[0] = scratch_.get();
@@ -446,6 +445,16 @@ int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms)
     });
 }
 
+int sph_solver_tuned_pass(sph_solver_t solver, int pass, int* variant, int* lines, double* ms) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(variant);
+        require_pointer(lines);
+        require_pointer(ms);
+        solver->impl->tuned_pass(pass, variant, lines, ms);
+    });
+}
+
 int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms) {
     return guarded([&] {
         require_handle(solver);
@@ -680,6 +689,15 @@ int sph_ensemble_time_passes(sph_ensemble_t ens, int reps, double* ms, double* b
         require_pointer(ms);
         require_pointer(bytes);
         ens->impl->time_passes(reps, ms, bytes);
+    });
+}
+
+int sph_ensemble_launch_mode(sph_ensemble_t ens, int* one_launch_per_pass, double* step_ms) {
+    return guarded([&] {
+        require_handle(ens);
+        require_pointer(one_launch_per_pass);
+        require_pointer(step_ms);
+        ens->impl->launch_mode(one_launch_per_pass, step_ms);
     });
 }
 

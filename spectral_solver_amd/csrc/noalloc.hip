@@ -110,9 +110,8 @@ void backward(const int shape[3], int precision, const void* spec, void* phys, h
     iz.out[0] = work;
     iz.nfields = 1;
     iz.ny = ny;
-    iz.n = nz;
     iz.tw = tw[2];
-    pt[2]->inv_z(iz, IZ_PLAIN, ny, H, s);
+    pt[2]->inv_z(iz, ny, H, s);
     pt[1]->lines(work, static_cast<long long>(H) * nz, true, tw[1], s);
     InvXArgs<T> ix{};
     ix.a[0] = work;

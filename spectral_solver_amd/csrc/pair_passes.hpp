@@ -122,9 +122,9 @@ template <int N, int NL>
 __host__ __device__ __forceinline__ int z_stash_at(int c, int e, int kyl) {
     return (c * N + e) * (NL / 2) + kyl;
 }
-template <typename T, int N>
+template <typename T, int N, int LINES = 0>
 constexpr bool z_stash_fits() {
-    return 3 * N * (ZTile<T, N>::NL / 2) <= z_lds<T, N>();
+    return 3 * N * (ZTile<T, N, LINES>::NL / 2) <= z_lds<T, N, LINES>();
 }
 
 // ---------------------------------------------------------------------------
@@ -134,10 +134,10 @@ constexpr bool z_stash_fits() {
 // ---------------------------------------------------------------------------
 // su (optional): the tile's state in LDS, written by pfwd_z_rk_tile (layout
 // z_stash_at); the state is then not read from global memory.
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int LINES = 0>
 __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
                                             int tid, const cx<T>* su = nullptr) {
-    using Z = ZTile<T, N>;
+    using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
     const int l = tid % NL, f = l & 1;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
             for (int i = 0; i < T0; ++i)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) v[0][i][r] = f ? cmul(us[c][i][r], ph[i][r]) : us[c][i][r];
-            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, LINES, z_pingpong<T, N, LINES>()>(v, w, lds, twl, tid);
             store(a.out[c]);
         }
     } else if constexpr (MODE == PZ_DIAG) {
@@ -230,7 +230,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
                     const cx<T> uc = c == 0 ? u0 : (c == 1 ? u1 : u2);
                     v[0][i][r] = f ? om : uc;
                 }
-            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, LINES, z_pingpong<T, N, LINES>()>(v, w, lds, twl, tid);
             store(a.out[c]);
         }
     } else {  // PZ_VEL: pairs (u0, u1) and (u2, 0)
@@ -241,42 +241,47 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
 #pragma unroll
                 for (int r = 0; r < R0; ++r)
                     v[0][i][r] = q == 0 ? (f ? us[1][i][r] : us[0][i][r]) : (f ? czero<T>() : us[2][i][r]);
-            zfft<T, N, 1, true, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
+            zfft<T, N, 1, true, LINES, z_pingpong<T, N, LINES>()>(v, w, lds, twl, tid);
             store(a.out[q]);
         }
     }
 }
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pinv_z(PairInvZArgs<T> a) {
-    using Z = ZTile<T, N>;
-    __shared__ cx<T> lds[z_lds<T, N>()];
+// (launch bounds: 512-thread tiles ask for 4 waves per SIMD, i.e. two
+// work-groups per CU, which caps the registers at 128 like the 1024-thread
+// tiles; smaller tiles leave the register budget to the compiler: held to
+// 128 the 96^3 forward-z kernel spills 240 bytes per lane)
+template <typename T, int N, int MODE, int LINES = 0>
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+    k_pinv_z(PairInvZArgs<T> a) {
+    using Z = ZTile<T, N, LINES>;
+    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
-    pinv_z_tile<T, N, MODE>(a, xcd_tile_id(), lds, twl, tid);
+    pinv_z_tile<T, N, MODE, LINES>(a, xcd_tile_id(), lds, twl, tid);
 }
 
 // ---------------------------------------------------------------------------
 // contiguous y lines of pair fields, in place: a tile is NL/2 pair lines,
 // i.e. NL*N consecutive complex values, member f of pair line g is line 2g+f
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool INV, int LINES = default_lines<T, N>()>
-__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
-    k_plines(cx<T>* data, long long npairlines, long long member_stride, const cx<T>* tw) {
+// One tile (the twiddles already in twl; buf holds tile_lds values):
+// pair lines tile.x*NL/2 .. of member tile.y.
+template <typename T, int N>
+constexpr bool plines_swizzled(int nl) {
+    return sizeof(T) == 4 && nl <= 8;  // narrow FP32 tiles: unpadded rotated rows (tile_at)
+}
+template <typename T, int N, bool INV, int LINES>
+__device__ __forceinline__ void plines_tile(cx<T>* data, long long npairlines, long long member_stride, TileId tile,
+                                            cx<T>* buf, const cx<T>* twl, int tid) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E;
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
-    // narrow FP32 tiles: unpadded rotated rows (conflict-free, tile_at)
-    constexpr bool SWZ = sizeof(T) == 4 && NL <= 8;
-    __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
-    __shared__ cx<T> twl[N];
-    const int tid = threadIdx.x;
-    const TileId tile = xcd_tile_id();  // an XCD's tiles are one contiguous run
+    constexpr bool SWZ = plines_swizzled<T, N>(NL);
     const long long g0 = static_cast<long long>(tile.x) * (NL / 2);
     cx<T>* base = data + tile.y * member_stride + g0 * 2 * N;
     const int valid = static_cast<int>(npairlines - g0 < NL / 2 ? npairlines - g0 : NL / 2) * 2 * N;
-    load_twiddles<T, N, NT>(twl, tw, tid);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         const int m = tid + i * NT;
@@ -291,6 +296,18 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
         const int g = m / (2 * N), rem = m % (2 * N);
         if (m < valid) base[m] = buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))];
     }
+}
+
+template <typename T, int N, bool INV, int LINES = default_lines<T, N>()>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
+    k_plines(cx<T>* data, long long npairlines, long long member_stride, const cx<T>* tw) {
+    using TL = Tile<T, N, LINES>;
+    __shared__ cx<T> buf[tile_lds<TL::NL, plines_swizzled<T, N>(TL::NL)>(N)];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    load_twiddles<T, N, TL::NT>(twl, tw, tid);
+    // an XCD's tiles are one contiguous run
+    plines_tile<T, N, INV, LINES>(data, npairlines, member_stride, xcd_tile_id(), buf, twl, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -531,12 +548,10 @@ constexpr int kVmaxStride = 16;
 // XT: the transposed layout [z][y][kx][2] in and out (k_plines_t): the tile's
 // NL lines are one contiguous block of NL * (N/2+1) pairs, row k of line l at
 // l * (N/2+1) + k.
+// One tile (lds holds z_lds<T, N, LINES>() values, the twiddles are in twl).
 template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
-// The second bound is the minimum number of waves per SIMD: 512-thread tiles
-// ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
-// at 128 like the 1024-thread form.
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
-    k_px_products_rd(PairXArgs<T> a) {
+__device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
+                                                    int tid) {
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
     constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
@@ -546,7 +561,6 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     // addressing pushes the 128-register x tile further into spills (3%
     // faster at 512; at 1024 the x pass takes 33.4 ms with it, 29.1 without)
     constexpr bool PP = N <= 512 && z_pingpong<T, N, LINES>();
-    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
     cx<T>* const sep = PP ? lds + Z::LDS : lds;
     // inverse transforms: first-stage butterflies in mirrored order, so the
     // two reads of each spectral row (elements k and N - k) come from one
@@ -558,12 +572,7 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     // wavefront and the pair separation is a lane exchange instead of a trip
     // through an LDS tile with two barriers
     constexpr bool MIRL = Z::kMirrorableLast;
-    __shared__ cx<T> twl[N];
-    const int tid = threadIdx.x;
     const int l = tid % NL;
-    // consecutive tiles of an XCD: neighbouring y blocks of one z, i.e. the
-    // same translation pages of every k row
-    const TileId tile = xcd_tile_id();
     const int y0 = tile.x * NL;
     const int z = tile.y;
     const bool ok = y0 + l < a.ny;
@@ -576,7 +585,6 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     constexpr uint32_t H = N / 2 + 1;
     // transposed layout: the tile's lines as one block (line l at l * H)
     const size_t xt_block = (static_cast<size_t>(z) * a.ny + y0) * H;
-    load_twiddles<T, N, NT>(twl, a.tw, tid);
     // u, s at slot m of element j + m*N/E (j = tid / NL)
     T u[3][E], s[3][E];
 #pragma unroll
@@ -718,6 +726,21 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     }
 }
 
+template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
+// The second bound is the minimum number of waves per SIMD: 512-thread tiles
+// ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
+// at 128 like the 1024-thread form.
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+    k_px_products_rd(PairXArgs<T> a) {
+    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    load_twiddles<T, N, ZTile<T, N, LINES>::NT>(twl, a.tw, tid);
+    // consecutive tiles of an XCD: neighbouring y blocks of one z, i.e. the
+    // same translation pages of every k row
+    px_products_rd_tile<T, N, LINES, VMAX, XT>(a, xcd_tile_id(), lds, twl, tid);
+}
+
 // LDS-staged form: each transform stages its whole tile through LDS.
 template <typename T, int N, int LINES = default_lines<T, N>()>
 __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXArgs<T> a) {
@@ -757,10 +780,10 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
 // ---------------------------------------------------------------------------
 // stash (optional): LDS for the updated state (z_stash_at), for the next
 // stage's inverse z in the same work-group.
-template <typename T, int N>
+template <typename T, int N, int LINES = 0>
 __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
                                                int tid, cx<T>* stash = nullptr) {
-    using Z = ZTile<T, N>;
+    using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
     constexpr int SL = TLS * RL;  // last-stage slots per lane
     constexpr int HS = SL / 2;    // modes owned by each lane of a line
@@ -826,7 +849,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 #pragma unroll
             for (int r = 0; r < R0; ++r) v[0][i][r] = vn[i][r];
         if (p < 5) load_pair(p + 1, vn);
-        zfft<T, N, 1, false, 0, z_pingpong<T, N>()>(v, w, lds, twl, tid);
+        zfft<T, N, 1, false, LINES, z_pingpong<T, N, LINES>()>(v, w, lds, twl, tid);
         // lane f = 0 holds P_hat, f = 1 holds Q_hat of every slot; swap halves
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
@@ -910,10 +933,11 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 // The inverse half writes the velocity pairs into
 // inv.out[0..2] (the first three product pair fields, whose values the
 // forward half has consumed: every work-group reads its own lines only).
-template <typename T, int N>
-__global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArgs<T> a, PairInvZArgs<T> inv) {
-    using Z = ZTile<T, N>;
-    __shared__ cx<T> lds[z_lds<T, N>()];
+template <typename T, int N, int LINES = 0>
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+    k_pfwd_z_rk_inv(PairFwdZArgs<T> a, PairInvZArgs<T> inv) {
+    using Z = ZTile<T, N, LINES>;
+    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const TileId tile = xcd_tile_id();
@@ -925,10 +949,10 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_pfwd_z_rk_inv(PairFwdZArg
     // acquire fences make every wave's state stores visible to the others
     // (one CU, one L1); a device-scope fence would write back the XCD's
     // whole L2 per work-group (measured: 6x slower pass).
-    constexpr bool kStash = z_stash_fits<T, N>();
-    pfwd_z_rk_tile<T, N>(a, tile, lds, twl, tid, kStash ? lds : nullptr);
+    constexpr bool kStash = z_stash_fits<T, N, LINES>();
+    pfwd_z_rk_tile<T, N, LINES>(a, tile, lds, twl, tid, kStash ? lds : nullptr);
     __syncthreads();
-    pinv_z_tile<T, N, PZ_RHS>(inv, tile, lds, twl, tid, kStash ? lds : nullptr);
+    pinv_z_tile<T, N, PZ_RHS, LINES>(inv, tile, lds, twl, tid, kStash ? lds : nullptr);
 }
 
 // max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)

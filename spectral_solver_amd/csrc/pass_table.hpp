@@ -7,14 +7,18 @@
 
 namespace sph {
 
-// Variants of the tuned passes (inverse y, x, forward y) that the solver
-// times on its own buffers, keeping the fastest (Group::autotune):
+// Variants of the tuned passes (inverse y, x, forward y, forward z + update +
+// next inverse z) that the solver times on its own buffers, keeping the
+// fastest (Group::autotune):
 //   y passes: LDS-staged tiles of the default width, half and a quarter of
 //             it (512^3 FP32 keeps the quarter: more, smaller work-groups per
 //             CU hide the latency of the tile loads better);
 //   x pass:   register-direct tiles of the default z-tile width, then half of
-//             it, then the LDS-staged form at the default width.
-constexpr int kTunedPasses = 3;
+//             it, then the LDS-staged form at the default width;
+//   forward z: the default z-tile width and half of it (half: twice the
+//             work-groups per CU where they fit; the standalone inverse z
+//             takes the same width).
+constexpr int kTunedPasses = 4;
 constexpr int kMaxVariants = 3;
 
 template <typename T>
@@ -24,13 +28,14 @@ struct PassTable {
     int nt;  // threads per work-group
     int variants[kTunedPasses];                // variants of each tuned pass
     int tile_lines[kTunedPasses][kMaxVariants];  // lines per tile of each variant
-    void (*inv_z)(const InvZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
+    void (*inv_z)(const InvZArgs<T>&, int nky, int nkx, hipStream_t);
     void (*lines)(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t);
     void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
-    void (*fwd_x)(const FwdXArgs<T>&, int mode, int ny, int nz, hipStream_t);
+    void (*fwd_x)(const FwdXArgs<T>&, int ny, int nz, hipStream_t);
     void (*fwd_z)(const FwdZArgs<T>&, int mode, int nky, int nkx, hipStream_t);
-    // solver stage on pair-interleaved buffers
-    void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, int members, hipStream_t);
+    // solver stage on pair-interleaved buffers (pinv_z, pfwd_z_rk: variant =
+    // z-tile width, as the forward-z pass's variants)
+    void (*pinv_z)(const PairInvZArgs<T>&, int mode, int nky, int nkx, int members, int variant, hipStream_t);
     void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                    const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
@@ -40,7 +45,8 @@ struct PassTable {
     // |u|^2 into a.vmax2 (variants 0 and 1; px_products_vmax)
     void (*px_products_vmax)(const PairXArgs<T>&, int variant, bool xt, hipStream_t);
     // forward z + update, then the next stage's inverse z (k_pfwd_z_rk_inv)
-    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, hipStream_t);
+    void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, int variant,
+                      hipStream_t);
     // transposed work layout between the y passes and the x pass (k_plines_t):
     // supported (xt), its y-pass tile widths (variants 0..2), and the pass
     // itself (line layout in and transposed out when inverse, else the other
@@ -63,21 +69,25 @@ const PassTable<T>* pass_table(int n);
 template <typename T, int N>
 void px_products_vmax(const PairXArgs<T>& a, int variant, bool xt, hipStream_t s);
 
-// The compiled transform lengths.
-#define SPH_LENGTHS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
+// The compiled transform lengths: powers of two and twice or four times them
+// times 3 up to 1024 (SPH_LENGTHS_A), and lengths with a factor 5 and the
+// rest of the 2^a 3 ones up to 768 (SPH_LENGTHS_B), compiled in translation
+// units of their own (passes_f32.hip, passes_f32_b.hip and the FP64 pair).
+#define SPH_LENGTHS_A(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(1024)
+#define SPH_LENGTHS_B(X) X(80) X(160) X(192) X(320) X(384) X(640) X(768)
+#define SPH_LENGTHS(X) SPH_LENGTHS_A(X) SPH_LENGTHS_B(X)
+
+// The pass tables of one translation unit's lengths (part 0: A, part 1: B).
+template <typename T, int PART>
+const PassTable<T>* pass_table_part(int n);
 
 template <typename T, int N>
 struct Launch {
     using TL = Tile<T, N>;
     using ZT = ZTile<T, N>;
-    static void inv_z(const InvZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
+    static void inv_z(const InvZArgs<T>& a, int nky, int nkx, hipStream_t s) {
         const dim3 grid((nky + ZT::NL - 1) / ZT::NL, nkx);
-        if (mode == IZ_RHS)
-            k_inv_z<T, N, IZ_RHS><<<grid, ZT::NT, 0, s>>>(a);
-        else if (mode == IZ_DIAG)
-            k_inv_z<T, N, IZ_DIAG><<<grid, ZT::NT, 0, s>>>(a);
-        else
-            k_inv_z<T, N, IZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
+        k_inv_z<T, N><<<grid, ZT::NT, 0, s>>>(a);
     }
     static void lines(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t s) {
         const dim3 grid(static_cast<unsigned>((nlines + TL::NL - 1) / TL::NL));
@@ -90,32 +100,38 @@ struct Launch {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
         k_inv_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
-    static void fwd_x(const FwdXArgs<T>& a, int mode, int ny, int nz, hipStream_t s) {
+    static void fwd_x(const FwdXArgs<T>& a, int ny, int nz, hipStream_t s) {
         const dim3 grid((ny + TL::NL - 1) / TL::NL, nz);
-        if (mode == FX_PRODUCTS)
-            k_fwd_x<T, N, FX_PRODUCTS><<<grid, TL::NT, 0, s>>>(a);
-        else
-            k_fwd_x<T, N, FX_PLAIN><<<grid, TL::NT, 0, s>>>(a);
+        k_fwd_x<T, N><<<grid, TL::NT, 0, s>>>(a);
     }
     static void fwd_z(const FwdZArgs<T>& a, int mode, int nky, int nkx, hipStream_t s) {
         const dim3 grid((nky + ZT::NL - 1) / ZT::NL, nkx);
-        if (mode == FZ_RK)
-            k_fwd_z<T, N, FZ_RK><<<grid, ZT::NT, 0, s>>>(a);
-        else if (mode == FZ_PROJECT)
+        if (mode == FZ_PROJECT)
             k_fwd_z<T, N, FZ_PROJECT><<<grid, ZT::NT, 0, s>>>(a);
         else
             k_fwd_z<T, N, FZ_PLAIN><<<grid, ZT::NT, 0, s>>>(a);
     }
-    // pair-buffer passes; `members` equal-size solves per launch (grid z, or
-    // grid y for the y lines)
-    static void pinv_z(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, hipStream_t s) {
-        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
+    // z tiles: the default width and half of it when that still makes a
+    // whole number of wavefronts
+    static constexpr int Z0 = ZT::NL;
+    static constexpr int Z1 = Z0 / 2 >= 2 && (Z0 / 2) * N / ZT::E >= 64 ? Z0 / 2 : Z0;
+    template <int L>
+    static void pinv_z_l(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, hipStream_t s) {
+        using ZV = ZTile<T, N, L>;
+        const dim3 grid((nky + ZV::NL / 2 - 1) / (ZV::NL / 2), nkx, members);
         if (mode == PZ_RHS)
-            k_pinv_z<T, N, PZ_RHS><<<grid, ZT::NT, 0, s>>>(a);
+            k_pinv_z<T, N, PZ_RHS, L><<<grid, ZV::NT, 0, s>>>(a);
         else if (mode == PZ_DIAG)
-            k_pinv_z<T, N, PZ_DIAG><<<grid, ZT::NT, 0, s>>>(a);
+            k_pinv_z<T, N, PZ_DIAG, L><<<grid, ZV::NT, 0, s>>>(a);
         else
-            k_pinv_z<T, N, PZ_VEL><<<grid, ZT::NT, 0, s>>>(a);
+            k_pinv_z<T, N, PZ_VEL, L><<<grid, ZV::NT, 0, s>>>(a);
+    }
+    static void pinv_z(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, int variant,
+                       hipStream_t s) {
+        if (variant == 1 && mode == PZ_RHS)
+            pinv_z_l<Z1>(a, mode, nky, nkx, members, s);
+        else
+            pinv_z_l<Z0>(a, mode, nky, nkx, members, s);
     }
     static constexpr int L0 = TL::NL;
     static constexpr int L1 = L0 / 2 >= 2 ? L0 / 2 : L0;
@@ -208,16 +224,25 @@ struct Launch {
             px_products_rd<X0>(a, ny, nz, members, s);
         }
     }
+    template <int L>
+    static void pfwd_z_rk_l(const PairFwdZArgs<T>& a, const PairInvZArgs<T>& inv, int nky, int nkx, int members,
+                            hipStream_t s) {
+        using ZV = ZTile<T, N, L>;
+        const dim3 grid((nky + ZV::NL / 2 - 1) / (ZV::NL / 2), nkx, members);
+        k_pfwd_z_rk_inv<T, N, L><<<grid, ZV::NT, 0, s>>>(a, inv);
+    }
     static void pfwd_z_rk(const PairFwdZArgs<T>& a, const PairInvZArgs<T>& inv, int nky, int nkx, int members,
-                          hipStream_t s) {
-        const dim3 grid((nky + ZT::NL / 2 - 1) / (ZT::NL / 2), nkx, members);
-        k_pfwd_z_rk_inv<T, N><<<grid, ZT::NT, 0, s>>>(a, inv);
+                          int variant, hipStream_t s) {
+        if (variant == 1)
+            pfwd_z_rk_l<Z1>(a, inv, nky, nkx, members, s);
+        else
+            pfwd_z_rk_l<Z0>(a, inv, nky, nkx, members, s);
     }
     static constexpr PassTable<T> table{N,
                                         TL::NL,
                                         TL::NT,
-                                        {3, 3, 3},
-                                        {{L0, L1, L2}, {X0, X1, L0}, {L0, L1, L2}},
+                                        {3, 3, 3, Z1 != Z0 ? 2 : 1},
+                                        {{L0, L1, L2}, {X0, X1, L0}, {L0, L1, L2}, {Z0, Z1, Z1}},
                                         &inv_z,
                                         &lines,
                                         &inv_x,

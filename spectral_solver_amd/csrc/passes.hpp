@@ -1,5 +1,6 @@
-// Pass kernels of the 3D real<->complex transforms, with the pointwise work of
-// the RK3 right-hand side folded into the first and last passes.
+// Kernels of the standalone 3D real<->complex transforms (the FFT backend
+// handle and the solver's initial condition), the projection and the
+// reductions shared with the solver's pass kernels (pair_passes.hpp).
 //
 // Storage (per field):
 //   physical  real  [z][y][x]            offset (z*ny + y)*nx + x
@@ -9,10 +10,7 @@
 //
 // Passes (inverse = spectral -> physical, forward = physical -> spectral):
 //   inv_z  : z lines of a tile of NL adjacent ky at fixed kx (128-byte row
-//            segments); RHS mode emits u_hat and the half-cell-shifted
-//            u_hat*phi (src/ava_taylor_green_impl.hpp:866-873) for 3
-//            components -> 6 fields; DIAG mode emits u_hat and
-//            omega_hat = i k x u_hat (impl.hpp:1147-1157).
+//            segments), any number of fields.
 //   lines  : y lines, contiguous, in place, any number of fields.
 //   inv_x  : x lines of NL adjacent y rows at fixed z; two half spectra A, B
 //            are packed as Z = A + iB with Hermitian extension (taking Re of
@@ -20,25 +18,19 @@
 //            reference's C2C backward keeps), inverse-transformed, and split
 //            into the real rows a = Re z, b = Im z, times 1/N^3
 //            (impl.hpp:555-561).
-//   fwd_x  : PRODUCTS mode forms the six products u_a*u_b and the six
-//            shifted products s_a*s_b (impl.hpp:903-906) in registers, packs
-//            each pair as p + i q, transforms, and separates the two half
-//            spectra: P = (Z_k + conj Z_{n-k})/2, Q = (Z_k - conj Z_{n-k})/(2i).
-//   fwd_z  : RK mode transforms the 12 product spectra along z and, per mode,
-//            applies the dealias combine and mask (impl.hpp:926-936), the -ik
-//            accumulation in the reference's order (00,01,02,11,12,22)
-//            (impl.hpp:881-886), the viscous term and RHS projection
-//            (impl.hpp:955-974), the Williamson update (impl.hpp:994-1033) and
-//            project_truncate of u_hat (impl.hpp:819-845), writing u_hat and g.
+//   fwd_x  : two real rows per complex transform, packed as p + i q,
+//            transformed, and separated into the two half spectra:
+//            P = (Z_k + conj Z_{n-k})/2, Q = (Z_k - conj Z_{n-k})/(2i).
+//   fwd_z  : z lines; PROJECT mode applies project_truncate
+//            (impl.hpp:819-845) to the three transformed components and may
+//            write them component-interleaved (the solver's state).
 #pragma once
 
 #include "fft_device.hpp"
 
 namespace sph {
 
-enum InvZMode { IZ_PLAIN = 0, IZ_RHS = 1, IZ_DIAG = 2 };
-enum FwdXMode { FX_PLAIN = 0, FX_PRODUCTS = 1 };
-enum FwdZMode { FZ_PLAIN = 0, FZ_PROJECT = 1, FZ_RK = 2 };
+enum FwdZMode { FZ_PLAIN = 0, FZ_PROJECT = 1 };
 
 __device__ __forceinline__ int signed_mode(int i, int n) { return i <= n / 2 ? i : i - n; }
 
@@ -107,12 +99,10 @@ __device__ __forceinline__ P pick(P const (&arr)[K], int q) {
 template <typename T>
 struct InvZArgs {
     const cx<T>* in[3];
-    cx<T>* out[6];
-    int nfields;         // PLAIN: in[f] -> out[f]
-    int ny;              // number of ky values (tile dimension)
-    int n;               // cube size for modes and phase (RHS, DIAG)
-    const cx<T>* phase;  // e^{i pi s/n}, indexed by s + n, s = mx+my+mz
-    const cx<T>* tw;     // twiddles of the z length
+    cx<T>* out[3];
+    int nfields;      // in[f] -> out[f]
+    int ny;           // number of ky values (tile dimension)
+    const cx<T>* tw;  // twiddles of the z length
 };
 
 template <typename T>
@@ -130,9 +120,9 @@ struct InvXArgs {
 
 template <typename T>
 struct FwdXArgs {
-    const T* in[6];
-    cx<T>* out[12];  // PRODUCTS: pair p -> out[2p] (plain), out[2p+1] (shifted)
-    int nin;         // PLAIN: 1..3 inputs, pairs (in0,in1) (in2,0) -> out[0..nin)
+    const T* in[3];
+    cx<T>* out[3];
+    int nin;  // 1..3 inputs, pairs (in0,in1) (in2,0) -> out[0..nin)
     int ny;
     int nz;
     const cx<T>* tw;
@@ -140,30 +130,23 @@ struct FwdXArgs {
 
 template <typename T>
 struct FwdZArgs {
-    const cx<T>* in[12];
+    const cx<T>* in[3];
     cx<T>* out[3];  // PLAIN / PROJECT outputs
     cx<T>* u3;      // PROJECT: if set, the output is this component-interleaved state
-    cx<T>* u[3];    // RK: state (read-write)
-    cx<T>* g[3];    // RK: low-storage register (read-write)
     int nfields;    // PLAIN
     int ny;
-    int n;
-    T nu;
-    const T* coef;  // RK: alpha_s, beta_s*dt per stage, in device memory
-    int stage;      // RK: 0, 1, 2
-    const cx<T>* phase;
+    int n;          // cube size for the modes (PROJECT)
     const cx<T>* tw;
 };
 
 // ---------------------------------------------------------------------------
 // inverse z pass (register-direct first/last stages, see zfft)
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
+template <typename T, int N>
 __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_inv_z(InvZArgs<T> a) {
     using Z = ZTile<T, N>;
     constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
-    constexpr int NF = MODE == IZ_PLAIN ? 1 : 2;
-    __shared__ cx<T> lds[NF * Z::LDS];
+    __shared__ cx<T> lds[Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int ky = blockIdx.x * NL + tid % NL;
@@ -190,57 +173,12 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_inv_z(InvZArgs<T> a) {
 #pragma unroll
             for (int r = 0; r < RL; ++r) b.st(lane + (i * Z::STEP + r * (N / RL)) * rowb, w[i][r]);
     };
-    cx<T> v[NF][T0][R0];
-    cx<T> w[NF][TLS][RL];
-    if constexpr (MODE == IZ_PLAIN) {
-        for (int f = 0; f < a.nfields; ++f) {
-            load(pick(a.in, f), v[0]);
-            zfft<T, N, 1, true>(v, w, lds, twl, tid);
-            store(pick(a.out, f), w[0]);
-        }
-    } else if constexpr (MODE == IZ_RHS) {
-        const int my = signed_mode(ky, a.n);
-#pragma unroll 1
-        for (int c = 0; c < 3; ++c) {
-            load(pick(a.in, c), v[0]);
-#pragma unroll
-            for (int i = 0; i < T0; ++i)
-#pragma unroll
-                for (int r = 0; r < R0; ++r) {
-                    const int mz = signed_mode(Z::e_first(tid, i, r), a.n);
-                    const cx<T> ph = a.phase[kx + my + mz + a.n];
-                    v[1][i][r] = cmul(v[0][i][r], ph);
-                }
-            zfft<T, N, 2, true>(v, w, lds, twl, tid);
-            store(pick(a.out, 2 * c), w[0]);
-            store(pick(a.out, 2 * c + 1), w[1]);
-        }
-    } else {  // IZ_DIAG: u and omega = i k x u
-        cx<T> u[3][T0][R0];
-        load(a.in[0], u[0]);
-        load(a.in[1], u[1]);
-        load(a.in[2], u[2]);
-        const T ix = T(kx), iy = T(signed_mode(ky, a.n));
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-#pragma unroll
-            for (int i = 0; i < T0; ++i)
-#pragma unroll
-                for (int r = 0; r < R0; ++r) {
-                    const T iz = T(signed_mode(Z::e_first(tid, i, r), a.n));
-                    const cx<T> u0 = u[0][i][r], u1 = u[1][i][r], u2 = u[2][i][r];
-                    v[0][i][r] = u[c][i][r];
-                    if (c == 0)
-                        v[1][i][r] = {-(iy * u2.y - iz * u1.y), iy * u2.x - iz * u1.x};
-                    else if (c == 1)
-                        v[1][i][r] = {-(iz * u0.y - ix * u2.y), iz * u0.x - ix * u2.x};
-                    else
-                        v[1][i][r] = {-(ix * u1.y - iy * u0.y), ix * u1.x - iy * u0.x};
-                }
-            zfft<T, N, 2, true>(v, w, lds, twl, tid);
-            store(a.out[2 * c], w[0]);
-            store(a.out[2 * c + 1], w[1]);
-        }
+    cx<T> v[1][T0][R0];
+    cx<T> w[1][TLS][RL];
+    for (int f = 0; f < a.nfields; ++f) {
+        load(pick(a.in, f), v[0]);
+        zfft<T, N, 1, true>(v, w, lds, twl, tid);
+        store(pick(a.out, f), w[0]);
     }
 }
 
@@ -359,7 +297,7 @@ __device__ __forceinline__ void fwd_x_separate_store(const cx<T>* buf, cx<T>* P,
     }
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N>
 __global__ void __launch_bounds__((Tile<T, N>::NT)) k_fwd_x(FwdXArgs<T> a) {
     using TL = Tile<T, N>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, S = TL::STRIDE;
@@ -381,49 +319,24 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_fwd_x(FwdXArgs<T> a) {
         off[i] = (z * a.ny + y0 + l) * N + x;
         li[i] = in[i] ? x * S + l : 0;
     }
-    if constexpr (MODE == FX_PRODUCTS) {
-        T u[3][E], s[3][E];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                u[c][i] = ok[i] ? a.in[c][off[i]] : T(0);
-                s[c][i] = ok[i] ? a.in[3 + c][off[i]] : T(0);
-            }
-        }
-        constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
-        constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-        for (int p = 0; p < 6; ++p) {
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                if (in[i]) buf[li[i]] = {u[PA[p]][i] * u[PB[p]][i], s[PA[p]][i] * s[PB[p]][i]};
-            }
-            __syncthreads();
-            fft_tile<T, N, NL, NT, false>(buf, twl, tid);
-            fwd_x_separate_store<T, N>(buf, a.out[2 * p], a.out[2 * p + 1], y0, z, a.ny, a.nz, tid);
-            __syncthreads();
-        }
-    } else {  // FX_PLAIN
 #pragma unroll 1
-        for (int p = 0; 2 * p < a.nin; ++p) {
-            const T* fa = pick(a.in, 2 * p);
-            const T* fb = 2 * p + 1 < a.nin ? pick(a.in, 2 * p + 1) : nullptr;
+    for (int p = 0; 2 * p < a.nin; ++p) {
+        const T* fa = pick(a.in, 2 * p);
+        const T* fb = 2 * p + 1 < a.nin ? pick(a.in, 2 * p + 1) : nullptr;
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                if (in[i]) buf[li[i]] = {ok[i] ? fa[off[i]] : T(0), (ok[i] && fb) ? fb[off[i]] : T(0)};
-            }
-            __syncthreads();
-            fft_tile<T, N, NL, NT, false>(buf, twl, tid);
-            fwd_x_separate_store<T, N>(buf, pick(a.out, 2 * p), fb ? pick(a.out, 2 * p + 1) : nullptr, y0, z, a.ny,
-                                       a.nz, tid);
-            __syncthreads();
+        for (int i = 0; i < E; ++i) {
+            if (in[i]) buf[li[i]] = {ok[i] ? fa[off[i]] : T(0), (ok[i] && fb) ? fb[off[i]] : T(0)};
         }
+        __syncthreads();
+        fft_tile<T, N, NL, NT, false>(buf, twl, tid);
+        fwd_x_separate_store<T, N>(buf, pick(a.out, 2 * p), fb ? pick(a.out, 2 * p + 1) : nullptr, y0, z, a.ny,
+                                   a.nz, tid);
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// forward z pass (+ fused RHS completion and RK3 update)
+// forward z pass (+ projection)
 // ---------------------------------------------------------------------------
 // Leray projection of f (impl.hpp:829-843): f -= k (k.f) * (1/k2) for kept
 // modes with k2 > 0, zero otherwise; evaluated without branches.
@@ -445,8 +358,7 @@ template <typename T, int N, int MODE>
 __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
     using Z = ZTile<T, N>;
     constexpr int NL = Z::NL, NT = Z::NT, T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
-    constexpr int NF = MODE == FZ_RK ? 2 : 1;
-    __shared__ cx<T> lds[NF * Z::LDS];
+    __shared__ cx<T> lds[Z::LDS];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const int ky = blockIdx.x * NL + tid % NL;
@@ -465,8 +377,8 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
             for (int r = 0; r < R0; ++r) v[i][r] = b.ld(lane + (i * Z::STEP + r * (N / R0)) * rowb);
     };
     auto last_off = [&](int i, int r) -> uint32_t { return lane + (i * Z::STEP + r * (N / RL)) * rowb; };
-    cx<T> v[NF][T0][R0];
-    cx<T> w[NF][TLS][RL];
+    cx<T> v[1][T0][R0];
+    cx<T> w[1][TLS][RL];
     if constexpr (MODE == FZ_PLAIN) {
         for (int f = 0; f < a.nfields; ++f) {
             load(pick(a.in, f), v[0]);
@@ -511,103 +423,6 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
                     }
                 }
         }
-    } else {  // FZ_RK
-        const int my = signed_mode(ky, a.n);
-        const T fx = T(kx), fy = T(my);
-        cx<T> r0[TLS][RL], r1[TLS][RL], r2[TLS][RL];
-#pragma unroll
-        for (int i = 0; i < TLS; ++i)
-#pragma unroll
-            for (int r = 0; r < RL; ++r) r0[i][r] = r1[i][r] = r2[i][r] = czero<T>();
-        // products in the reference's accumulation order (00,01,02,11,12,22)
-#pragma unroll 1
-        for (int p = 0; p < 6; ++p) {
-            const int pa = p < 3 ? 0 : (p < 5 ? 1 : 2);
-            const int pb = p < 3 ? p : (p < 5 ? p - 2 : 2);
-#if SPH_RK_PAIRED || !defined(SPH_RK_PAIRED)
-            load(pick(a.in, 2 * p), v[0]);
-            load(pick(a.in, 2 * p + 1), v[1]);
-            zfft<T, N, 2, false>(v, w, lds, twl, tid);
-#else
-            {
-                cx<T> v1[1][T0][R0], w1[1][TLS][RL];
-                load(pick(a.in, 2 * p), v1[0]);
-                zfft<T, N, 1, false>(v1, w1, lds, twl, tid);
-#pragma unroll
-                for (int i = 0; i < TLS; ++i)
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) w[0][i][r] = w1[0][i][r];
-                load(pick(a.in, 2 * p + 1), v1[0]);
-                zfft<T, N, 1, false>(v1, w1, lds, twl, tid);
-#pragma unroll
-                for (int i = 0; i < TLS; ++i)
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) w[1][i][r] = w1[0][i][r];
-            }
-#endif
-#pragma unroll
-            for (int i = 0; i < TLS; ++i)
-#pragma unroll
-                for (int r = 0; r < RL; ++r) {
-                    const int mz = signed_mode(Z::e_last(tid, i, r), a.n);
-                    const int k2 = kx * kx + my * my + mz * mz;
-                    const cx<T> ph = a.phase[kx + my + mz + a.n];
-                    cx<T> val = (w[0][i][r] + cmul(w[1][i][r], cx<T>{ph.x, -ph.y})) * T(0.5);
-                    if (!keep_mode(k2, a.n)) val = czero<T>();
-                    const T fz = T(mz);
-                    const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
-                    const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
-                    const cx<T> da = {kb * val.y, -kb * val.x};  // -i k_b v into r_a
-                    const cx<T> db = {ka * val.y, -ka * val.x};  // -i k_a v into r_b
-                    if (pa == 0)
-                        r0[i][r] = r0[i][r] + da;
-                    else if (pa == 1)
-                        r1[i][r] = r1[i][r] + da;
-                    else
-                        r2[i][r] = r2[i][r] + da;
-                    if (pa != pb) {
-                        if (pb == 1)
-                            r1[i][r] = r1[i][r] + db;
-                        else
-                            r2[i][r] = r2[i][r] + db;
-                    }
-                }
-        }
-        if (!ok) return;
-        const T alpha = a.coef[2 * a.stage];
-        const T beta = a.coef[2 * a.stage + 1];
-        const bool first = a.stage == 0;
-        const Buf<T> U[3] = {Buf<T>(a.u[0] + plane, plane_bytes), Buf<T>(a.u[1] + plane, plane_bytes),
-                             Buf<T>(a.u[2] + plane, plane_bytes)};
-        const Buf<T> G[3] = {Buf<T>(a.g[0] + plane, plane_bytes), Buf<T>(a.g[1] + plane, plane_bytes),
-                             Buf<T>(a.g[2] + plane, plane_bytes)};
-#pragma unroll
-        for (int i = 0; i < TLS; ++i)
-#pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                const uint32_t off = last_off(i, r);
-                const int mz = signed_mode(Z::e_last(tid, i, r), a.n);
-                const int k2 = kx * kx + my * my + mz * mz;
-                const bool keep = keep_mode(k2, a.n);
-                cx<T> u[3] = {U[0].ld(off), U[1].ld(off), U[2].ld(off)};
-                cx<T> rr[3] = {r0[i][r], r1[i][r], r2[i][r]};
-                const T vk2 = a.nu * T(k2);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) rr[c] = rr[c] - u[c] * vk2;
-                project_or_zero(rr[0], rr[1], rr[2], kx, my, mz, k2, keep);
-                cx<T> g[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    g[c] = first ? rr[c] : G[c].ld(off) * alpha + rr[c];
-                    u[c] = u[c] + g[c] * beta;
-                }
-                project_or_zero(u[0], u[1], u[2], kx, my, mz, k2, keep);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    G[c].st(off, g[c]);
-                    U[c].st(off, u[c]);
-                }
-            }
     }
 }
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ensemble_passes.hpp"
 #include "spectral_hip.h"
 
 namespace sph {
@@ -167,6 +168,8 @@ public:
         // one member at a time
         P_.resize(6 * npts);
         coef_.resize(static_cast<std::size_t>(count) * kCoefStride);
+        // the autotuner's timing launches of the forward-z pass read them
+        hip_check(hipMemset(coef_.get(), 0, count * kCoefStride * sizeof(T)), "hipMemset");
         partial_.resize(3 * kReduceBlocks + 8);
         result_.resize(8);
         upload(tw_, twiddle_table<T>(n));
@@ -189,13 +192,18 @@ public:
     // not timed or not defined)
     // (in the layout kept: the transposed layout's variants and times when
     // it was chosen)
+    // tuning: the first three tuned passes (inverse y, x, forward y);
+    // tuned_pass: any of the four
     void tuning(int* variant, int* lines, double* ms) const {
-        for (int p = 0; p < kTuned; ++p) {
-            const int v = xt_ ? xvar_[p] : var_[p];
-            variant[p] = v;
-            lines[p] = xt_ && p != 1 ? pt_->xt_lines[v] : pt_->tile_lines[p][v];
-            for (int q = 0; q < kMaxVariants; ++q) ms[p * kMaxVariants + q] = xt_ ? xtune_ms_[p][q] : tune_ms_[p][q];
-        }
+        for (int p = 0; p < 3; ++p) tuned_pass(p, variant + p, lines + p, ms + p * kMaxVariants);
+    }
+    void tuned_pass(int p, int* variant, int* lines, double* ms) const {
+        if (p < 0 || p >= kTuned) throw std::runtime_error("tuned pass out of range");
+        const bool xt = xt_ && p < 3;  // the forward-z pass is the same in both layouts
+        const int v = xt ? xvar_[p] : var_[p];
+        *variant = v;
+        *lines = xt && p != 1 ? pt_->xt_lines[v] : pt_->tile_lines[p][v];
+        for (int q = 0; q < kMaxVariants; ++q) ms[q] = xt ? xtune_ms_[p][q] : tune_ms_[p][q];
     }
     // the work layout kept (1: transposed between the y passes and the x
     // pass) and the summed time of passes 1-3 with each layout's fastest
@@ -245,6 +253,54 @@ public:
         return a;
     }
 
+    // Arguments of the x pass (pair fields at src, member 0) and of the
+    // forward-z pass (state u, g and pair fields W of member `first`).
+    PairXArgs<T> x_args(cx<T>* src) const {
+        const std::int64_t M2 = 2 * modes;
+        PairXArgs<T> a{};
+        for (int c = 0; c < 3; ++c) a.ab[c] = src + c * M2;
+        for (int q = 0; q < 6; ++q) a.out[q] = src + q * M2;
+        a.ny = n;
+        a.nz = n;
+        a.scale = T(1) / static_cast<T>(npts);
+        a.tw = tw_.get();
+        a.ms_w = sw;
+        return a;
+    }
+    PairFwdZArgs<T> fwd_z_args(cx<T>* u, cx<T>* g, cx<T>* W, int first, int stage) const {
+        const std::int64_t M2 = 2 * modes;
+        PairFwdZArgs<T> a{};
+        for (int q = 0; q < 6; ++q) a.in[q] = W + q * M2;
+        a.u3 = u;
+        a.g3 = g;
+        a.ny = n;
+        a.n = n;
+        a.coef = coef_.get() + first * kCoefStride;
+        a.stage = stage;
+        a.phase = phase_.get();
+        a.tw = tw_.get();
+        a.ms_u = su;
+        a.ms_w = sw;
+        return a;
+    }
+
+    // Everything the one-launch-per-pass ensemble kernels need of this group
+    // (every member; ensemble_passes.hpp).
+    EnsGroupArgs<T> ens_args() const {
+        EnsGroupArgs<T> e{};
+        cx<T>* W = W_.get();
+        e.n = n;
+        e.tw = tw_.get();
+        e.inv = inv_z_args(U_.get(), W);
+        for (int st = 0; st < 3; ++st) e.fwd[st] = fwd_z_args(U_.get(), G_.get(), W, 0, st);
+        e.x = x_args(W);
+        e.lines = W;
+        e.npairlines[0] = 3LL * H * n;
+        e.npairlines[1] = 6LL * H * n;
+        e.member_stride = sw;
+        return e;
+    }
+
     // pass k (0 inverse z, 1 inverse y, 2 inverse x + products + forward x,
     // 3 forward y, 4 forward z + update + the next stage's inverse z) of
     // stage `stage` for members [first, first + members); u and g address
@@ -264,7 +320,7 @@ public:
         cx<T>* X = xt_ ? X_.get() + first * sw : nullptr;
         switch (k) {
             case 0:
-                pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, s);
+                pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, var_[3], s);
                 break;
             case 1:
                 if (xt_)
@@ -273,15 +329,7 @@ public:
                     pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
             case 2: {
-                cx<T>* src = xt_ ? X : W;
-                PairXArgs<T> a{};
-                for (int c = 0; c < 3; ++c) a.ab[c] = src + c * M2;
-                for (int q = 0; q < 6; ++q) a.out[q] = src + q * M2;
-                a.ny = n;
-                a.nz = n;
-                a.scale = T(1) / static_cast<T>(npts);
-                a.tw = tw_.get();
-                a.ms_w = sw;
+                PairXArgs<T> a = x_args(xt_ ? X : W);
                 a.vmax2 = vmax2;
                 const int xv = xt_ ? xvar_[1] : var_[1];
                 if (vmax2)
@@ -297,19 +345,8 @@ public:
                     pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
                 break;
             default: {
-                PairFwdZArgs<T> a{};
-                for (int q = 0; q < 6; ++q) a.in[q] = W + q * M2;
-                a.u3 = u;
-                a.g3 = g;
-                a.ny = n;
-                a.n = n;
-                a.coef = coef_.get() + first * kCoefStride;
-                a.stage = stage;
-                a.phase = phase_.get();
-                a.tw = tw_.get();
-                a.ms_u = su;
-                a.ms_w = sw;
-                pt_->pfwd_z_rk(a, inv_z_args(u, W), n, H, members, s);
+                const PairFwdZArgs<T> a = fwd_z_args(u, g, W, first, stage);
+                pt_->pfwd_z_rk(a, inv_z_args(u, W), n, H, members, var_[3], s);
                 break;
             }
         }
@@ -372,7 +409,7 @@ public:
         x.scale = T(1) / static_cast<T>(npts);
         x.tw = tw_.get();
         if (mode == PZ_DIAG) {
-            pt_->pinv_z(a, PZ_DIAG, n, H, 1, s);
+            pt_->pinv_z(a, PZ_DIAG, n, H, 1, 0, s);
             pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             for (int c = 0; c < 3; ++c) {
                 x.ab[c] = W + c * M2;
@@ -381,7 +418,7 @@ public:
             }
             x.npairs = 3;
         } else {
-            pt_->pinv_z(a, PZ_VEL, n, H, 1, s);
+            pt_->pinv_z(a, PZ_VEL, n, H, 1, 0, s);
             pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             x.ab[0] = W;
             x.oa[0] = P;
@@ -483,7 +520,7 @@ public:
         fx.ny = n;
         fx.nz = n;
         fx.tw = tw_.get();
-        pt_->fwd_x(fx, FX_PLAIN, n, n, s);
+        pt_->fwd_x(fx, n, n, s);
         pt_->lines(W, 3LL * H * n, false, tw_.get(), s);
         FwdZArgs<T> fz{};
         for (int c = 0; c < 3; ++c) fz.in[c] = W + c * M;
@@ -565,7 +602,7 @@ private:
         if (force && force[0] >= '0' && force[0] < '0' + kMaxVariants && force[1] == '\0') {
             for (int p = 0; p < kTuned; ++p) {
                 var_[p] = std::min(force[0] - '0', pt_->variants[p] - 1);
-                xvar_[p] = std::min(force[0] - '0', p == 1 ? 1 : kMaxVariants - 1);
+                xvar_[p] = p == 3 ? 0 : std::min(force[0] - '0', p == 1 ? 1 : kMaxVariants - 1);
             }
             use_transposed(forced_layout == 1 && xt_allowed);
             return;
@@ -602,12 +639,14 @@ private:
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "hipEventCreate");
         hip_check(hipEventCreate(&b), "hipEventCreate");
-        const int pass_of[kTuned] = {1, 2, 3};
-        // every variant of the tuned passes in the current layout; returns the
-        // summed time of the fastest
+        const int pass_of[kTuned] = {1, 2, 3, 4};
+        // every variant of the tuned passes in the current layout (nvar[p]
+        // of pass p; 0: not tuned here); returns the summed time of the
+        // fastest of passes 1-3, the ones the layouts differ in
         auto tune_layout = [&](int (&var)[kTuned], double (&ms)[kTuned][kMaxVariants], const int* nvar) {
             double total = 0.0;
             for (int p = 0; p < kTuned; ++p) {
+                if (nvar[p] < 1) continue;
                 int best = 0;
                 for (int v = 0; v < nvar[p]; ++v) {
                     var[p] = v;
@@ -622,7 +661,7 @@ private:
                     if (ms[p][v] < ms[p][best]) best = v;
                 }
                 var[p] = best;
-                total += ms[p][best];
+                if (p < 3) total += ms[p][best];
             }
             return total;
         };
@@ -631,7 +670,7 @@ private:
         bool keep_xt = false;
         if (xt_allowed) {
             use_transposed(true);
-            const int nx[kTuned] = {kMaxVariants, 2, kMaxVariants};
+            const int nx[kTuned] = {kMaxVariants, 2, kMaxVariants, 0};
             layout_ms_[1] = tune_layout(xvar_, xtune_ms_, nx);
             keep_xt = forced_layout == 1 || layout_ms_[1] < layout_ms_[0];
         }
@@ -788,6 +827,7 @@ public:
     virtual void step_adaptive(double dt_max, double cfl, double* vmax, double* dt_used) = 0;
     virtual bool fused_velocity_max() const = 0;
     virtual void tuning(int* variant, int* lines, double* ms) const = 0;
+    virtual void tuned_pass(int pass, int* variant, int* lines, double* ms) const = 0;
     virtual void layout(int* transposed, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
@@ -952,6 +992,9 @@ public:
     }
 
     void tuning(int* variant, int* lines, double* ms) const override { g_.tuning(variant, lines, ms); }
+    void tuned_pass(int pass, int* variant, int* lines, double* ms) const override {
+        g_.tuned_pass(pass, variant, lines, ms);
+    }
     void layout(int* transposed, double* ms) const override { g_.layout(transposed, ms); }
 
     // One adaptive step as the driver takes it (impl.hpp:1363-1378, without
@@ -1002,6 +1045,7 @@ public:
     virtual void synchronize() = 0;
     virtual double step_seconds(long long* steps) = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
+    virtual void launch_mode(int* fused, double* ms) const = 0;
     std::vector<int> sizes;
 };
 
@@ -1030,6 +1074,11 @@ public:
         for (std::size_t q = 0; q < group_n_.size(); ++q)
             groups_.push_back(std::make_unique<Group<T>>(group_n_[q], static_cast<int>(group_nu_[q].size()),
                                                          group_nu_[q]));
+        dt_.resize(groups_.size());
+        for (std::size_t q = 0; q < groups_.size(); ++q) dt_[q].assign(groups_[q]->count, 0.0);
+        make_branches();
+        // the states are still zero: time the two launch structures on them
+        make_fused();
         for (int j = 0; j < count; ++j) {
             const int nj = n[j];
             const std::size_t modes = static_cast<std::size_t>(nj / 2 + 1) * nj * nj;
@@ -1039,9 +1088,6 @@ public:
             const auto w = locate(j);
             groups_[w.first]->set_state(w.second, u, stream_.get());
         }
-        dt_.resize(groups_.size());
-        for (std::size_t q = 0; q < groups_.size(); ++q) dt_[q].assign(groups_[q]->count, 0.0);
-        make_branches();
     }
 
     void step(const double* dt) override {
@@ -1050,23 +1096,7 @@ public:
         for (std::size_t q = 0; q < groups_.size(); ++q) groups_[q]->set_dt(dt_[q].data(), s);
         auto ev = timer_.begin(s);
         for (auto& g : groups_) g->prepare_pairs(s);
-        // one branch per size: the groups' chains of 12 launches are
-        // independent, so small grids run beside large ones
-        auto record = [&] {
-            if (groups_.size() == 1) {
-                for (int st = 0; st < 3; ++st) groups_[0]->launch_stage(st, s);
-                return;
-            }
-            hip_check(hipEventRecord(fork_, s), "hipEventRecord");
-            for (std::size_t q = 0; q < groups_.size(); ++q) {
-                const hipStream_t b = branch_[q]->get();
-                hip_check(hipStreamWaitEvent(b, fork_, 0), "hipStreamWaitEvent");
-                for (int st = 0; st < 3; ++st) groups_[q]->launch_stage(st, b);
-                hip_check(hipEventRecord(join_[q], b), "hipEventRecord");
-            }
-            for (std::size_t q = 0; q < groups_.size(); ++q)
-                hip_check(hipStreamWaitEvent(s, join_[q], 0), "hipStreamWaitEvent");
-        };
+        auto record = [&] { record_step(s); };
         if (use_graph_) {
             graph_.launch(s, record);
         } else {
@@ -1079,6 +1109,11 @@ public:
     sph_diagnostics diagnostics(int member, double t) override {
         const auto w = locate(member);
         return groups_[w.first]->diagnostics(w.second, t, stream_.get());
+    }
+    void launch_mode(int* fused, double* ms) const override {
+        *fused = use_fused_ ? 1 : 0;
+        ms[0] = mode_ms_[0];
+        ms[1] = mode_ms_[1];
     }
     double max_velocity(int member) override {
         const auto w = locate(member);
@@ -1118,6 +1153,84 @@ private:
         return where_[member];
     }
 
+    // The three stages of a step. Groups in the one-launch-per-pass set run as
+    // four launches per stage on the main stream; every other group runs its
+    // own chain of four launches per stage on a branch of its own (fork/join
+    // through events), so small grids run beside large ones either way.
+    void record_step(hipStream_t s) {
+        std::vector<std::size_t> own;
+        for (std::size_t q = 0; q < groups_.size(); ++q)
+            if (!(use_fused_ && is_fused(q))) own.push_back(q);
+        const bool fused = use_fused_ && !fused_.empty();
+        if (!fused && own.size() == 1) {
+            for (int st = 0; st < 3; ++st) groups_[own[0]]->launch_stage(st, s);
+            return;
+        }
+        if (!own.empty()) {
+            hip_check(hipEventRecord(fork_, s), "hipEventRecord");
+            for (std::size_t q : own) {
+                const hipStream_t b = branch_[q]->get();
+                hip_check(hipStreamWaitEvent(b, fork_, 0), "hipStreamWaitEvent");
+                for (int st = 0; st < 3; ++st) groups_[q]->launch_stage(st, b);
+                hip_check(hipEventRecord(join_[q], b), "hipEventRecord");
+            }
+        }
+        if (fused)
+            for (int st = 0; st < 3; ++st)
+                for (int k = 1; k < Group<T>::kPasses; ++k) ens_launch<T>(k, table_.get(), grid_[k], st, s);
+        for (std::size_t q : own) hip_check(hipStreamWaitEvent(s, join_[q], 0), "hipStreamWaitEvent");
+    }
+
+    bool is_fused(std::size_t q) const { return std::find(fused_.begin(), fused_.end(), q) != fused_.end(); }
+
+    // One launch per pass for the groups of the sizes ensemble_passes.hpp
+    // fuses (largest first), and the choice between that and one chain per
+    // group: SPH_ENSEMBLE_FUSED=0 / 1 forces it, otherwise both are timed on
+    // the still-zero states (one warm-up and two timed steps each, launched
+    // directly) and the faster is kept.
+    void make_fused() {
+        for (std::size_t q = 0; q < groups_.size(); ++q)
+            if (ens_size(groups_[q]->n) && static_cast<int>(fused_.size()) < kEnsGroups) fused_.push_back(q);
+        if (fused_.empty()) return;
+        std::sort(fused_.begin(), fused_.end(), [&](std::size_t a, std::size_t b) { return groups_[a]->n > groups_[b]->n; });
+        std::vector<EnsGroupArgs<T>> args;
+        std::vector<int> fn, fm;
+        for (std::size_t q : fused_) {
+            args.push_back(groups_[q]->ens_args());
+            fn.push_back(groups_[q]->n);
+            fm.push_back(groups_[q]->count);
+        }
+        table_.resize(args.size());
+        hip_check(hipMemcpy(table_.get(), args.data(), args.size() * sizeof(args[0]), hipMemcpyHostToDevice),
+                  "hipMemcpy ensemble table");
+        for (int k = 0; k < Group<T>::kPasses; ++k)
+            grid_[k] = ens_grid<T>(k, static_cast<int>(fused_.size()), fn.data(), fm.data());
+        const char* env = std::getenv("SPH_ENSEMBLE_FUSED");
+        if (env && (env[0] == '0' || env[0] == '1') && env[1] == '\0') {
+            use_fused_ = env[0] == '1';
+            return;
+        }
+        const hipStream_t s = stream_.get();
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        for (int mode = 0; mode < 2; ++mode) {
+            use_fused_ = mode == 1;
+            record_step(s);
+            hip_check(hipEventRecord(a, s), "hipEventRecord");
+            for (int r = 0; r < 2; ++r) record_step(s);
+            hip_check(hipEventRecord(b, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float e = 0.f;
+            hip_check(hipEventElapsedTime(&e, a, b), "hipEventElapsedTime");
+            mode_ms_[mode] = e / 2.0;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        hip_check(hipGetLastError(), "ensemble launch timing");
+        use_fused_ = mode_ms_[1] < mode_ms_[0];
+    }
+
     void make_branches() {
         if (groups_.size() < 2) return;
         hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
@@ -1139,6 +1252,15 @@ private:
     std::vector<std::unique_ptr<Stream>> branch_;
     hipEvent_t fork_ = nullptr;
     std::vector<hipEvent_t> join_;
+    // one launch per pass: the fused groups (indices into groups_), their
+    // argument table on the device, the grid of each pass, whether in use,
+    // and the timed step of each launch structure (ms: per-group chains,
+    // one launch per pass)
+    std::vector<std::size_t> fused_;
+    DeviceArray<EnsGroupArgs<T>> table_;
+    EnsGrid grid_[Group<T>::kPasses] = {};
+    bool use_fused_ = false;
+    double mode_ms_[2] = {0.0, 0.0};
 
 public:
     ~Ensemble() override {

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_supported_lengths():
     lib = sph.load_library()
-    for n in (8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024):
+    for n in (8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024):
         assert lib.sph_supported_length(n) == 1
     for n in (0, 1, 7, 100, 2048):
         assert lib.sph_supported_length(n) == 0

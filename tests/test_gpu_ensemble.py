@@ -147,3 +147,54 @@ def test_config_64_members_100_steps():
         assert err < 2e-5, (j, err)
         de = abs(ens.diagnostics(j).dissipation - o.diagnostics(0.0)["dissipation"])
         assert de <= 1e-4 * o.diagnostics(0.0)["dissipation"], j
+
+
+def test_one_launch_per_pass_matches_oracle(monkeypatch):
+    """Every fused size (16, 32, 48, 64, 96, 128) in one launch per pass,
+    plus a size outside that set (24) on its own chain beside them; FP64
+    members against their oracle solves."""
+    monkeypatch.setenv("SPH_ENSEMBLE_FUSED", "1")
+    sizes = [128, 16, 24, 32, 48, 64, 96, 32]
+    nus = [0.01, 0.02, 0.01, 0.005, 0.01, 0.02, 0.01, 0.03]
+    seeds = [2000 + j for j in range(len(sizes))]
+    print("case ensemble one launch per pass fp64", sizes, flush=True)
+    ens = sph.Ensemble(sizes, nus, seeds, precision="fp64")
+    assert ens.launch_mode()[0]
+    orcs = [member_oracle(n, nu, s) for n, nu, s in zip(sizes, nus, seeds)]
+    dt = [1e-3 * (1 + 0.1 * j) for j in range(len(sizes))]
+    for _ in range(3):
+        ens.step_rk3(dt)
+        for o, d in zip(orcs, dt):
+            o.step_rk3(float(d))
+    for j, o in enumerate(orcs):
+        err = member_error(ens, j, o)
+        print(f"  member {j} ({sizes[j]}^3): rel L2 vs oracle {err:.2e}", flush=True)
+        assert err < 1e-12, (j, err)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("fp64", 1e-14)])
+def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision, tol):
+    sizes = [32, 48, 64, 96] * 2
+    seeds = [3000 + j for j in range(len(sizes))]
+    print("case ensemble launch structures", precision, flush=True)
+    states = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("SPH_ENSEMBLE_FUSED", mode)
+        ens = sph.Ensemble(sizes, 1e-2, seeds, precision=precision)
+        assert ens.launch_mode()[0] == (mode == "1")
+        dt = cfl_dt(ens)
+        for _ in range(4):
+            ens.step_rk3(dt)
+        states.append([[ens.u_hat(j, c) for c in range(3)] for j in range(len(sizes))])
+        ens.close()
+    for j in range(len(sizes)):
+        num = sum(np.linalg.norm(states[0][j][c] - states[1][j][c]) ** 2 for c in range(3))
+        den = sum(np.linalg.norm(states[1][j][c]) ** 2 for c in range(3))
+        assert np.sqrt(num / den) <= tol, (j, np.sqrt(num / den))
+
+
+def test_launch_mode_reports_both_timings():
+    ens = sph.Ensemble([32, 48, 64, 96], 1e-2, [1, 2, 3, 4], precision="fp32")
+    fused, ms = ens.launch_mode()
+    print("  one launch per pass:", fused, "step ms per-size chains / one launch per pass:", ms, flush=True)
+    assert ms[0] > 0 and ms[1] > 0 and fused == (ms[1] < ms[0])

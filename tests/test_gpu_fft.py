@@ -17,7 +17,10 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(16, 16, 16), (12, 12, 12), (32, 16, 8), (8, 24, 48), (64, 64, 64), (96, 32, 16), (128, 8, 12),
           (16, 8, 96), (24, 12, 24),
           # the x and z passes' long plans (1024 = 8*4*4*8, 256 = 8*4*8) along x and z
-          (1024, 8, 12), (12, 8, 1024), (256, 16, 256)]
+          (1024, 8, 12), (12, 8, 1024), (256, 16, 256),
+          # lengths with a factor 5, and 2^a 3 beyond 96 (radix-4 z/x plans
+          # with a radix 2 or 6 and the 3s / 5s: 20 or 12 values per thread)
+          (80, 16, 160), (320, 8, 192), (16, 384, 80), (640, 12, 8), (12, 8, 768), (768, 12, 640)]
 
 
 def half_of(c2c):

@@ -77,7 +77,7 @@ def random_state(n, seed):
     return o, u
 
 
-@pytest.mark.parametrize("n,steps", [(32, 3), (48, 2), (24, 2)])
+@pytest.mark.parametrize("n,steps", [(32, 3), (48, 2), (24, 2), (80, 2), (160, 1), (192, 1)])
 def test_random_state_with_nyquist_modes_matches_oracle(n, steps):
     print("case random", n, flush=True)
     o, u = random_state(n, 5 + n)
@@ -264,7 +264,7 @@ def test_cli_fixed_dt(tmp_path):
     assert r.returncode == 1 and "error: grid size must be positive" in r.stderr
 
 
-@pytest.mark.parametrize("precision,n", [("fp64", 32), ("fp32", 64)])
+@pytest.mark.parametrize("precision,n", [("fp64", 32), ("fp32", 64), ("fp64", 128)])
 def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
     """Every autotuner variant of the y and x passes (tile widths, and the
     register-direct and LDS-staged x pass) gives the oracle's state (FP64:
@@ -272,7 +272,8 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
     compiler contracts multiply-adds differently in different kernels, so the
     bits may differ). In FP32 at 64^3 the 8- and 4-line y tiles use the
     unpadded rotated LDS rows and the 8-line x tile the unpadded z-tile rows
-    with the permuted first-stage write."""
+    with the permuted first-stage write; variants 1 and 2 also run the
+    forward-z pass (and the standalone inverse z) on half-width z tiles."""
     steps = 3
     o, u = random_state(n, 11)
     states = []
@@ -281,7 +282,12 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
         monkeypatch.setenv("SPH_TILE_VARIANT", v)
         g = sph.TaylorGreenSolver(n, 400.0, precision)
         k = int(v)
-        assert [t[0] for t in g.tuning().values()] == [k, k, k]
+        tv = g.tuning()
+        assert [tv[p][0] for p in ("inverse_y", "x_products", "forward_y")] == [k, k, k]
+        # the forward-z pass has two widths where the half tile still fills
+        # whole wavefronts (64^3 FP32: 16 and 8 lines; 128^3 FP64: 8 and 4),
+        # else one
+        assert tv["forward_z_update"][0] == (min(k, 1) if (precision, n) in (("fp32", 64), ("fp64", 128)) else 0)
         for c in range(3):
             g.set_u_hat(c, oracle.c2c_to_half(u[c]))
         for _ in range(steps):
