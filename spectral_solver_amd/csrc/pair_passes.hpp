@@ -191,21 +191,24 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
                 for (int c = 0; c < 3; ++c) us[c][i][r] = U.ld(uoff(i, r, c));
     }
     if constexpr (MODE == PZ_RHS) {
-        // member 0: u_hat_c, member 1: u_hat_c * phi (impl.hpp:866-873)
-        cx<T> ph[T0][R0];
+        // member 0: u_hat_c, member 1: u_hat_c * phi (impl.hpp:866-873); the
+        // shift is applied to all three components before the first
+        // transform, so the phases need no registers during the transforms
 #pragma unroll
         for (int i = 0; i < T0; ++i)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
                 const int mz = signed_mode(Z::e_first(tid, i, r), a.n);
-                ph[i][r] = f ? a.phase[kx + my + mz + a.n] : cx<T>{T(1), T(0)};
+                const cx<T> ph = f ? a.phase[kx + my + mz + a.n] : cx<T>{T(1), T(0)};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) us[c][i][r] = f ? cmul(us[c][i][r], ph) : us[c][i][r];
             }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
 #pragma unroll
             for (int i = 0; i < T0; ++i)
 #pragma unroll
-                for (int r = 0; r < R0; ++r) v[0][i][r] = f ? cmul(us[c][i][r], ph[i][r]) : us[c][i][r];
+                for (int r = 0; r < R0; ++r) v[0][i][r] = us[c][i][r];
             zfft<T, N, 1, true, LINES, z_pingpong<T, N, LINES>()>(v, w, lds, twl, tid);
             store(a.out[c]);
         }
@@ -812,15 +815,17 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
     // per owned mode: signed kz, |k|^2, mask and the conjugate phase, looked up
     // once (not per product pair), and the stage coefficients, so that no
     // dependent global load sits between a transform and its accumulation
-    int mzh[HS], k2h[HS];
-    bool keeph[HS];
+    // (four registers per mode: kz as T, the combine's factor 1/2 or 0 for a
+    // masked mode, the conjugate phase; the integer |k|^2 and mask are formed
+    // again for the update)
+    T fzh[HS], halfh[HS];
     cx<T> phc[HS];
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
-        mzh[h] = signed_mode(owned_e(h), a.n);
-        k2h[h] = kx * kx + my * my + mzh[h] * mzh[h];
-        keeph[h] = keep_mode(k2h[h], a.n);
-        const cx<T> ph = a.phase[kx + my + mzh[h] + a.n];
+        const int mz = signed_mode(owned_e(h), a.n);
+        fzh[h] = T(mz);
+        halfh[h] = keep_mode(kx * kx + my * my + mz * mz, a.n) ? T(0.5) : T(0);
+        const cx<T> ph = a.phase[kx + my + mz + a.n];
         phc[h] = {ph.x, -ph.y};
     }
     const T alpha = coef[2 * a.stage];
@@ -858,9 +863,10 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
             const cx<T> other = shfl_xor1(f ? lo : hi);
             const cx<T> P = f ? other : mine;
             const cx<T> Q = f ? mine : other;
-            cx<T> val = (P + cmul(Q, phc[h])) * T(0.5);
-            if (!keeph[h]) val = czero<T>();
-            const T fz = T(mzh[h]);
+            // 0.5 (P + Q conj(phi)), zero for a masked mode (its accumulators
+            // are zeroed by the projection below whatever they hold)
+            const cx<T> val = (P + cmul(Q, phc[h])) * halfh[h];
+            const T fz = fzh[h];
             const T kb = pb == 0 ? fx : (pb == 1 ? fy : fz);
             const T ka = pa == 0 ? fx : (pa == 1 ? fy : fz);
             const cx<T> da = {kb * val.y, -kb * val.x};  // -i k_b v into r_a
@@ -896,8 +902,9 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
         for (int c = 0; c < 3; ++c) g[h][c] = first ? czero<T>() : G.ld(off[h] + c * CB);
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
-        const int mz = mzh[h], k2 = k2h[h];
-        const bool keep = keeph[h];
+        const int mz = static_cast<int>(fzh[h]);
+        const int k2 = kx * kx + my * my + mz * mz;
+        const bool keep = keep_mode(k2, a.n);
         cx<T> rr[3] = {r0[h], r1[h], r2[h]};
         const T vk2 = nu * T(k2);
 #pragma unroll
