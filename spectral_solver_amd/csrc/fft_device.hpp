@@ -17,8 +17,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
 namespace sph {
 
 template <typename T>
@@ -622,88 +620,6 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
                 dft<RL, INV>(w[f][i]);
             }
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Last radix-8 stage of a forward transform on the matrix cores (FP32;
-// 16-line tiles with one last-stage task per thread in mirrored order).
-// The 64 lanes of a wavefront are four lane rows of 16 (g = lane / 16,
-// line l = lane % 16); lane row b holds butterfly j_b = ZTile::j_mirror of
-// its lane group in the vector form. Here the eight-point DFTs of one j for
-// the 16 lines are one real product D (16 x 16) = F (16 x 16) X (16 x 16),
-// four v_mfma_f32_16x16x4f32 with K = 4 each, for each of the wavefront's
-// four j. Operand layout of that instruction (one block): lane L supplies
-// A[L % 16][L / 16] and B[L / 16][L % 16], and accumulator r of lane L is
-// D[4 (L / 16) + r][L % 16]. Lane (g, l) supplies, for K step m, the twiddled
-// input x_g (m = 0, 1: real, imaginary part) and x_{g+4} (m = 2, 3) of
-// butterfly (j, l): B row k = 4m + g is (input g + 4 (m >> 1), part m & 1).
-// D row i is output q = i / 4 + 4 ((i % 4) >> 1), part i & 1, so lane (g, l)
-// receives {Re X_g, Im X_g, Re X_{g+4}, Im X_{g+4}} of butterfly (j, l).
-// ---------------------------------------------------------------------------
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-
-// Entry F[i][k] of the forward eight-point DFT in the operand order above
-// (k = 4m + g): the lane's four A operands, am[m] = F[lane % 16][4m + lane / 16].
-__host__ __device__ inline void dft8_mfma_operands(int lane, float (&am)[4]) {
-    const float h = 0.707106781186547524400844362104849039f;
-    const int i = lane & 15, g = lane >> 4;
-    const int q = (i >> 2) + 4 * ((i & 3) >> 1), pout = i & 1;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int r = g + 4 * (m >> 1), pin = m & 1;
-        const int t = (q * r) & 7;  // w^(q r), w = e^{-2 pi i / 8}
-        // cos and sin of -2 pi t / 8
-        const float c = t == 0 ? 1.f : (t == 4 ? -1.f : (t == 2 || t == 6 ? 0.f : (t == 1 || t == 7 ? h : -h)));
-        const float sn = t == 2 ? -1.f : (t == 6 ? 1.f : (t == 0 || t == 4 ? 0.f : (t < 4 ? -h : h)));
-        // Re X = c Re x - s Im x, Im X = s Re x + c Im x
-        am[m] = pout == 0 ? (pin == 0 ? c : -sn) : (pin == 0 ? sn : c);
-    }
-}
-
-// Pair separation after the stage: lane (g, l) of lane row b writes row
-// e = j_b + g STEP (g < 4, so e < N/2; wavefront 0 also writes N/2, element
-// 4 STEP of j = 0), which needs output N - e of the same line. That value is
-// slot `slot` (0: X_{g'}, 1: X_{g'+4}) of lane row g' of the butterfly in
-// lane row `b` of the same wavefront: row b ^ 1 holds j' = STEP - j
-// (j_mirror pairs lane groups 2m and 2m+1), and N - e = j' + (7 - g) STEP,
-// i.e. slot 1 of lane row g ^ 3; in wavefront 0, rows 0 and 1 hold j = 0 and
-// j = STEP / 2, which are their own mirrors (N - e = (8 - g) mod 8 STEP, and
-// STEP / 2 + (7 - g) STEP).
-struct Mfma8Partner {
-    int b, g, slot;
-};
-__host__ __device__ inline Mfma8Partner mfma8_partner(bool wave0, int b, int g) {
-    if (wave0 && b == 0) return g == 0 ? Mfma8Partner{0, 0, 0} : Mfma8Partner{0, (4 - g) & 3, 1};
-    if (wave0 && b == 1) return {1, g ^ 3, 1};
-    return {b ^ 1, g ^ 3, 1};
-}
-
-// The stage for lane rows 2h and 2h + 1 of this wavefront (a pair of
-// mirrored butterflies, or in wavefront 0 for h = 0 the two self-mirrored
-// ones, so the pair separation needs no other rows): last is the tile
-// zfft_head returned, am from dft8_mfma_operands; d[c] receives the outputs
-// of lane row 2h + c's butterfly as above.
-template <int N, int LINES>
-__device__ __forceinline__ void zlast_mfma8(const cx<float>* last, const cx<float>* twl, const float (&am)[4],
-                                            int tid, int h, mfma_f4 (&d)[2]) {
-    using Z = ZTile<float, N, LINES>;
-    static_assert(Z::NL == 16 && Z::RL == 8 && Z::kMirrorableLast && Z::NT % 64 == 0,
-                  "matrix-core last stage: 16-line tiles, radix-8 last stage, one task per thread");
-    constexpr int S = Z::S, STEP = Z::STEP;
-    const int lane = tid & 63, g = lane >> 4, l = lane & 15;
-    const int p0 = (tid >> 6) * 4 + 2 * h;  // lane group of lane row 2h
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int j = Z::j_mirror((p0 + c) * Z::NL);
-        const cx<float> x1 = cmul(last[(j + g * STEP) * S + l], twl[g * j]);
-        const cx<float> x2 = cmul(last[(j + (g + 4) * STEP) * S + l], twl[(g + 4) * j]);
-        mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(am[0], x1.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(am[1], x1.y, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(am[2], x2.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(am[3], x2.y, acc, 0, 0, 0);
-        d[c] = acc;
     }
 }
 

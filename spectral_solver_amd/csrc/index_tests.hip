@@ -2,7 +2,6 @@
 // and every permutation of butterflies or slots must be a bijection, or a
 // kernel would silently overwrite or skip values. Built by build.py, run by
 // tests/test_index_maps.py (no GPU needed).
-#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -124,72 +123,6 @@ void xor_orders() {
 
 }  // namespace
 
-// The matrix-core last stage (zlast_mfma8) and its pair separation, with
-// the instruction's operand layout emulated on the host: lane L gives
-// A[L % 16][4m + L / 16] and B[4m + L / 16][L % 16] at K step m, and holds
-// D[4 (L / 16) + r][L % 16] in accumulator r. The product must be the
-// eight-point DFT (checked against the definition in double), and every
-// row k = 0 .. N/2 must be written once with its partner N - k.
-template <int N, int LINES>
-void mfma8_stage() {
-    using Z = sph::ZTile<float, N, LINES>;
-    static_assert(Z::NL == 16 && Z::RL == 8 && Z::kMirrorableLast && Z::NT % 64 == 0, "not an MFMA tile");
-    constexpr int STEP = Z::STEP;
-    float am[64][4];
-    for (int L = 0; L < 64; ++L) sph::dft8_mfma_operands(L, am[L]);
-    unsigned state = 12345u;
-    auto rnd = [&]() {
-        state = state * 1664525u + 1013904223u;
-        return static_cast<double>(state >> 8) / (1u << 24) - 0.5;
-    };
-    for (int trial = 0; trial < 4; ++trial) {
-        double xr[16][8], xi[16][8];  // column (line) c, input r, twiddled
-        for (int c = 0; c < 16; ++c)
-            for (int r = 0; r < 8; ++r) {
-                xr[c][r] = static_cast<float>(rnd());
-                xi[c][r] = static_cast<float>(rnd());
-            }
-        double D[16][16] = {};
-        for (int m = 0; m < 4; ++m)
-            for (int L = 0; L < 64; ++L) {  // lane L supplies A[L%16][4m+L/16], B[4m+L/16][L%16]
-                const int g = L / 16, c = L % 16, r = g + 4 * (m >> 1);
-                const double bval = (m & 1) ? xi[c][r] : xr[c][r];
-                for (int i = 0; i < 16; ++i) D[i][c] += static_cast<double>(am[i + 16 * g][m]) * bval;
-            }
-        const double kTwoPi = 6.283185307179586476925286766559;
-        for (int L = 0; L < 64; ++L) {
-            const int g = L / 16, c = L % 16;
-            for (int r = 0; r < 4; ++r) {
-                const int q = g + 4 * (r >> 1);
-                double re = 0, im = 0;
-                for (int k = 0; k < 8; ++k) {
-                    const double a = -kTwoPi * q * k / 8;
-                    re += std::cos(a) * xr[c][k] - std::sin(a) * xi[c][k];
-                    im += std::sin(a) * xr[c][k] + std::cos(a) * xi[c][k];
-                }
-                const double want = (r & 1) ? im : re, got = D[4 * g + r][c];
-                expect(std::fabs(want - got) < 1e-6, "mfma8 dft", L, r);
-            }
-        }
-    }
-    std::vector<int> seen(N / 2 + 1, 0);
-    for (int w = 0; w < Z::NT / 64; ++w)
-        for (int b = 0; b < 4; ++b) {
-            const int j = Z::j_mirror((4 * w + b) * Z::NL);
-            for (int g = 0; g < 4; ++g) {
-                const int e = j + g * STEP;
-                expect(e <= N / 2, "mfma8 row range", w, e);
-                if (e <= N / 2) ++seen[e];
-                const sph::Mfma8Partner pt = sph::mfma8_partner(w == 0, b, g);
-                const int jp = Z::j_mirror((4 * w + pt.b) * Z::NL);
-                const int ep = jp + (pt.g + 4 * pt.slot) * STEP;
-                expect(ep == (N - e) % N, "mfma8 partner", e, ep);
-            }
-        }
-    ++seen[N / 2];  // wavefront 0, lane row 0: element 4 STEP of j = 0
-    for (int k = 0; k <= N / 2; ++k) expect(seen[k] == 1, "mfma8 rows once", N, k);
-}
-
 int main() {
     for (int n : {8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024}) {
         swizzled_tile<2>(n);
@@ -224,8 +157,6 @@ int main() {
     mirror_last<double, 256, 0>();
     mirror_last<double, 512, 0>();
     xor_orders();
-    mfma8_stage<256, 16>();
-    mfma8_stage<512, 16>();
     if (failures) {
         std::printf("%d index-map checks failed\n", failures);
         return 1;

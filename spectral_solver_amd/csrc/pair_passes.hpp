@@ -551,11 +551,8 @@ constexpr int kVmaxStride = 16;
 // XT: the transposed layout [z][y][kx][2] in and out (k_plines_t): the tile's
 // NL lines are one contiguous block of NL * (N/2+1) pairs, row k of line l at
 // l * (N/2+1) + k.
-// MF: the forward transforms' last radix-8 stage on the matrix cores
-// (zlast_mfma8; FP32, 16-line tiles), the pair separation from its output
-// layout.
 // One tile (lds holds z_lds<T, N, LINES>() values, the twiddles are in twl).
-template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false, bool MF = false>
+template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
                                                     int tid) {
     using Z = ZTile<T, N, LINES>;
@@ -659,10 +656,6 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
         if (tid % 64 == 0)
             atomicMax(a.vmax2 + slot * kVmaxStride, static_cast<unsigned long long>(__double_as_longlong(m)));
     }
-    static_assert(!MF || (std::is_same_v<T, float> && NL == 16 && RL == 8 && MIRL),
-                  "matrix-core last stage: FP32 16-line tiles with a mirrored radix-8 last stage");
-    float am[4];
-    if constexpr (MF) dft8_mfma_operands(tid & 63, am);
     constexpr int PA[6] = {0, 0, 0, 1, 1, 2};
     constexpr int PB[6] = {0, 1, 2, 1, 2, 2};
     constexpr int ITEMS = NL * (N / 2 + 1);
@@ -677,45 +670,6 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 const int m = i + r * T0;
                 v[0][i][r] = {u[PA[p]][m] * u[PB[p]][m], s[PA[p]][m] * s[PB[p]][m]};
             }
-        if constexpr (MF) {
-            const cx<T>* const last = zfft_head<T, N, 1, false, LINES, PP, false>(v, lds, twl, tid);
-            // lane (g, l) writes row j_b + g STEP of each lane row b; its
-            // partner N - k comes from lane row pt.g (mfma8_partner) of the
-            // same pair of lane rows
-            const int g = (tid & 63) >> 4;
-            const bool w0 = tid < 64;
-            cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw) +
-                        (XT ? xt_block + static_cast<size_t>(l) * H : static_cast<size_t>(z) * a.ny + y0 + l);
-            const size_t ostep = XT ? 1 : row;
-            auto separate = [](cx<T> zk, cx<T> zm) {
-                cx2<T> q;
-                q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-                q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                return q;
-            };
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                mfma_f4 d[2];
-                zlast_mfma8<N, LINES>(last, twl, am, tid, h, d);
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int b = 2 * h + c;
-                    const int j = Z::j_mirror(((tid >> 6) * 4 + b) * NL);
-                    const Mfma8Partner pt = mfma8_partner(w0, b, g);
-                    const mfma_f4 src = (h == 0 && w0) ? d[c] : d[c ^ 1];  // lane row pt.b
-                    const int sl = pt.g * 16 + l;
-                    cx<T> zm = {__shfl(src[2], sl), __shfl(src[3], sl)};
-                    if (pt.slot == 0) zm = {src[0], src[1]};
-                    const cx2<T> q = separate(cx<T>{d[c][0], d[c][1]}, zm);
-                    if (ok) o[static_cast<size_t>(j + g * Z::STEP) * ostep] = q;
-                }
-                if (h == 0 && w0 && g == 0 && ok) {  // row N/2: element 4 STEP of j = 0, its own partner
-                    const cx<T> zh = {d[0][2], d[0][3]};
-                    o[static_cast<size_t>(N / 2) * ostep] = separate(zh, zh);
-                }
-            }
-            continue;
-        }
         zfft<T, N, 1, false, LINES, PP, false, MIRL>(v, w, lds, twl, tid);
         if constexpr (MIRL) {
             // element j + r*STEP for r < RL/2 (and N/2 in group j = 0): rows
@@ -775,7 +729,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     }
 }
 
-template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false, bool MF = false>
+template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 // The second bound is the minimum number of waves per SIMD: 512-thread tiles
 // ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
 // at 128 like the 1024-thread form.
@@ -787,7 +741,7 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
     load_twiddles<T, N, ZTile<T, N, LINES>::NT>(twl, a.tw, tid);
     // consecutive tiles of an XCD: neighbouring y blocks of one z, i.e. the
     // same translation pages of every k row
-    px_products_rd_tile<T, N, LINES, VMAX, XT, MF>(a, xcd_tile_id(), lds, twl, tid);
+    px_products_rd_tile<T, N, LINES, VMAX, XT>(a, xcd_tile_id(), lds, twl, tid);
 }
 
 // LDS-staged form: each transform stages its whole tile through LDS.

@@ -3,8 +3,6 @@
 // passes_f32.hip / passes_f64.hip.
 #pragma once
 
-#include <cstdlib>
-
 #include "pair_passes.hpp"
 
 namespace sph {
@@ -22,17 +20,6 @@ namespace sph {
 //             takes the same width).
 constexpr int kTunedPasses = 4;
 constexpr int kMaxVariants = 3;
-
-// SPH_X_MFMA=1: variant 0 of the x pass (line layout) runs its forward
-// transforms' last radix-8 stage on the matrix cores where the tile allows
-// it (FP32, 16-line tiles with a mirrored radix-8 last stage).
-inline bool x_mfma_requested() {
-    static const bool on = [] {
-        const char* e = std::getenv("SPH_X_MFMA");
-        return e && e[0] == '1' && e[1] == '\0';
-    }();
-    return on;
-}
 
 template <typename T>
 struct PassTable {
@@ -176,16 +163,12 @@ struct Launch {
     // still makes a whole number of wavefronts
     static constexpr int X0 = ZT::NL;
     static constexpr int X1 = X0 / 2 >= 2 && (X0 / 2) * N / ZT::E >= 64 ? X0 / 2 : X0;
-    template <int L, bool XT = false, bool MF = false>
+    template <int L, bool XT = false>
     static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
         const dim3 grid((ny + L - 1) / L, nz, members);
-        k_px_products_rd<T, N, L, false, XT, MF><<<grid, ZV::NT, 0, s>>>(a);
+        k_px_products_rd<T, N, L, false, XT><<<grid, ZV::NT, 0, s>>>(a);
     }
-    // the default-width x tile with its forward transforms' last stage on the
-    // matrix cores (zlast_mfma8), where the tile allows it
-    static constexpr bool kMF = std::is_same_v<T, float> && ZTile<T, N, X0>::NL == 16 &&
-                                ZTile<T, N, X0>::RL == 8 && ZTile<T, N, X0>::kMirrorableLast;
     // the transposed work layout (k_plines_t, k_px_products_rd<..., XT>) at
     // the lengths where the x pass's rows lie far apart
     static constexpr bool kXT = N >= 256;
@@ -238,12 +221,6 @@ struct Launch {
         } else if (variant == 1) {
             px_products_rd<X1>(a, ny, nz, members, s);
         } else {
-            if constexpr (kMF) {
-                if (x_mfma_requested()) {
-                    px_products_rd<X0, false, true>(a, ny, nz, members, s);
-                    return;
-                }
-            }
             px_products_rd<X0>(a, ny, nz, members, s);
         }
     }
