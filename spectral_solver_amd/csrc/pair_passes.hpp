@@ -590,6 +590,17 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     const size_t xt_block = (static_cast<size_t>(z) * a.ny + y0) * H;
     // u, s at slot m of element j + m*N/E (j = tid / NL)
     T u[3][E], s[3][E];
+    // Where this pass runs without ping-pong tiles but the kernel allocates
+    // them (N >= 1024), component 0's u and s are parked in the unused second
+    // tile after its inverse transform: 2E registers fewer through the other
+    // transforms (at 1024^3 the 128-register FP32 tiles spilled 40-52 bytes
+    // per lane to scratch, about 14 GB of HBM traffic per launch). Each
+    // thread reads back only its own slots.
+    constexpr bool PARK = !PP && z_pingpong<T, N, LINES>();
+    static_assert(!PARK || E * NT <= Z::LDS, "the parking area fits the second tile");
+    cx<T>* const park = lds + Z::LDS;
+    auto uval = [&](int c, int m) -> T { return PARK && c == 0 ? park[m * NT + tid].x : u[c][m]; };
+    auto sval = [&](int c, int m) -> T { return PARK && c == 0 ? park[m * NT + tid].y : s[c][m]; };
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + (XT ? xt_block : static_cast<size_t>(z) * a.ny + y0);
@@ -641,6 +652,12 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 u[c][i + r * TLS] = w[0][i][r].x * a.scale;
                 s[c][i + r * TLS] = w[0][i][r].y * a.scale;
             }
+        if constexpr (PARK) {
+            if (c == 0) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) park[m * NT + tid] = {u[0][m], s[0][m]};
+            }
+        }
     }
     if constexpr (VMAX) {
         // |u|^2 in double as the diagnostics' reduction forms it (k_reduce_phys);
@@ -648,7 +665,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
         double m = 0.0;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            const double p = u[0][i], q = u[1][i], r = u[2][i];
+            const double p = uval(0, i), q = u[1][i], r = u[2][i];
             m = fmax(m, p * p + q * q + r * r);
         }
         m = wave_max(m);
@@ -668,7 +685,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
                 const int m = i + r * T0;
-                v[0][i][r] = {u[PA[p]][m] * u[PB[p]][m], s[PA[p]][m] * s[PB[p]][m]};
+                v[0][i][r] = {uval(PA[p], m) * uval(PB[p], m), sval(PA[p], m) * sval(PB[p], m)};
             }
         zfft<T, N, 1, false, LINES, PP, false, MIRL>(v, w, lds, twl, tid);
         if constexpr (MIRL) {
