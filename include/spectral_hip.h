@@ -199,22 +199,38 @@ int sph_solver_step_adaptive(sph_solver_t solver, double dt_max, double cfl, dou
 
 /* The choices of the solver's autotuner (the reference plans its FFTs with
  * FFTW_ESTIMATE / cuFFT, impl.hpp:257-412; here the y and x passes come in
- * several tile widths and kernel forms, timed at creation on the solver's own
- * buffers). For the tuned passes inverse-y, x, forward-y: variant[3] = the
- * variant kept, lines[3] = its lines per tile, ms[9] = mean launch time of
- * each variant, {pass0 v0, pass0 v1, pass0 v2, pass1 v0, ...} (0 where a pass
- * has fewer variants, or tuning was switched off with SPH_AUTOTUNE=0).
- * Variants: y passes 0 = default width, 1 = half width; x pass
- * 0 = register-direct default width, 1 = register-direct half width,
- * 2 = LDS-staged default width. */
+ * several tile widths, kernel forms and radix orders, timed at creation on
+ * the solver's own buffers). For the tuned passes inverse-y, x, forward-y:
+ * variant[3] = the variant kept, lines[3] = its lines per tile, ms[12] = mean
+ * launch time of each variant, {pass0 v0, .. pass0 v3, pass1 v0, ...} (0
+ * where a pass has fewer variants, or tuning was switched off with
+ * SPH_AUTOTUNE=0). Variants: y passes 0 = default width, 1 = half width,
+ * 2 = a quarter; x pass 0 = register-direct default width, 1 =
+ * register-direct half width, 2 = LDS-staged default width, 3 =
+ * register-direct default width in the alternative radix order (lengths that
+ * have one, sph_radix_plan). */
 int sph_solver_tuning(sph_solver_t solver, int* variant, int* lines, double* ms);
 
 /* One tuned pass of the autotuner, any of the four: pass 0 inverse y, 1 x,
  * 2 forward y (as sph_solver_tuning), 3 forward z + update + next inverse z
- * (variants: 0 = default z-tile width, 1 = half of it; the standalone inverse
- * z takes the same width). variant, lines per tile, ms[3] = mean launch time
- * of each variant (0 where the pass has fewer variants). */
+ * (variants: 0 = default z-tile width, 1 = half of it, 2 = default width in
+ * the alternative radix order; the standalone inverse z takes the same width
+ * and order). variant, lines per tile, ms[4] = mean launch time of each
+ * variant (0 where the pass has fewer variants). */
 int sph_solver_tuned_pass(sph_solver_t solver, int pass, int* variant, int* lines, double* ms);
+
+/* The radix stages of the transforms in the variant the autotuner kept for
+ * tuned pass `pass` (0..3 as sph_solver_tuned_pass), first stage first:
+ * radices[0 .. *count-1], radices holds 16 values. */
+int sph_solver_radix_plan(sph_solver_t solver, int pass, int* radices, int* count);
+
+/* The radix stages of the z-type passes (z transforms, register-direct x
+ * pass) for transform length n: order 0 = the default plan, 1 = the
+ * alternative order the autotuner times beside it (the reference leaves plan
+ * choice to FFTW / cuFFT, src/fftw_fft.cpp:31-32, impl.hpp:288,300-301).
+ * radices holds 16 values; *count = 0 when n is not a compiled length. No GPU
+ * needed. */
+int sph_radix_plan(int n, int order, int* radices, int* count);
 
 /* The work layout the autotuner kept. Between the inverse-y pass, the x pass
  * and the forward-y pass the pair fields are either in the line layout

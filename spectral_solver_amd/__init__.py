@@ -92,6 +92,8 @@ SYMBOLS = [
     ("sph_solver_tuning", ctypes.c_int, [_VP, _IP, _IP, _DP]),
     ("sph_solver_layout", ctypes.c_int, [_VP, _IP, _DP]),
     ("sph_solver_tuned_pass", ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP, _DP]),
+    ("sph_solver_radix_plan", ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP]),
+    ("sph_radix_plan", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _IP, _IP]),
     ("sph_measure_copy_bandwidth", ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _DP]),
     ("sph_run_taylor_green", ctypes.c_int, [ctypes.POINTER(_RunConfig), ctypes.POINTER(_Diag)]),
     ("sph_ensemble_create", ctypes.c_int, [ctypes.c_int, _IP, _DP, ctypes.POINTER(ctypes.c_uint64),
@@ -133,6 +135,19 @@ def _check(rc: int) -> None:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(_DP)
+
+
+# variants per tuned pass in the autotuner's reports (kMaxVariants)
+MAX_VARIANTS = 4
+
+
+def radix_plan(n: int, order: int = 0) -> list:
+    """Radix stages of the z-type passes for transform length n: order 0 the
+    default plan, 1 the alternative order the autotuner also times ([] when n
+    is not a compiled length). Needs no GPU."""
+    r, c = np.zeros(16, np.int32), ctypes.c_int()
+    _check(load_library().sph_radix_plan(int(n), int(order), r.ctypes.data_as(_IP), ctypes.byref(c)))
+    return [int(x) for x in r[:c.value]]
 
 
 def device_count() -> int:
@@ -398,14 +413,25 @@ class TaylorGreenSolver:
         each variant)."""
         var = np.zeros(3, np.int32)
         lines = np.zeros(3, np.int32)
-        ms = np.zeros(9)
+        ms = np.zeros(3 * MAX_VARIANTS)
         _check(load_library().sph_solver_tuning(self._h, var.ctypes.data_as(_IP), lines.ctypes.data_as(_IP),
                                                 _dptr(ms)))
         names = ("inverse_y", "x_products", "forward_y")
-        out = {k: (int(var[p]), int(lines[p]), [float(x) for x in ms[3 * p:3 * p + 3]]) for p, k in enumerate(names)}
-        v, l, m3 = ctypes.c_int(), ctypes.c_int(), np.zeros(3)
-        _check(load_library().sph_solver_tuned_pass(self._h, 3, ctypes.byref(v), ctypes.byref(l), _dptr(m3)))
-        out["forward_z_update"] = (v.value, l.value, [float(x) for x in m3])
+        V = MAX_VARIANTS
+        out = {k: (int(var[p]), int(lines[p]), [float(x) for x in ms[V * p:V * p + V]]) for p, k in enumerate(names)}
+        v, l, m4 = ctypes.c_int(), ctypes.c_int(), np.zeros(V)
+        _check(load_library().sph_solver_tuned_pass(self._h, 3, ctypes.byref(v), ctypes.byref(l), _dptr(m4)))
+        out["forward_z_update"] = (v.value, l.value, [float(x) for x in m4])
+        return out
+
+    def radix_plans(self) -> dict:
+        """Radix stages of the transforms in the kept variant of each tuned
+        pass (first stage first)."""
+        out = {}
+        for p, k in enumerate(("inverse_y", "x_products", "forward_y", "forward_z_update")):
+            r, c = np.zeros(16, np.int32), ctypes.c_int()
+            _check(load_library().sph_solver_radix_plan(self._h, p, r.ctypes.data_as(_IP), ctypes.byref(c)))
+            out[k] = [int(x) for x in r[:c.value]]
         return out
 
     def u_hat(self, c: int) -> np.ndarray:

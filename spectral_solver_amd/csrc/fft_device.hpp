@@ -93,7 +93,12 @@ constexpr bool length_supported(int n) { return n >= 2 && radix_plan(n).count > 
 // Powers of two with two or more radix-8 stages put one of them last (1024 =
 // 8*4*4*8, 256 = 8*4*8): the same stages, and the last one then holds one
 // task per thread (ZTile::kMirrorableLast).
-constexpr RadixPlan z_radix_plan(int n) {
+//
+// order = 1 is the alternative stage order the autotuner times beside it
+// (z_radix_plan_alt).
+constexpr RadixPlan z_radix_plan_alt(int n);
+constexpr RadixPlan z_radix_plan(int n, int order = 0) {
+    if (order == 1) return z_radix_plan_alt(n);
     const RadixPlan p = radix_plan(n);
     if (p.count < 0) return p;
     bool has3 = false, has5 = false;
@@ -138,6 +143,56 @@ constexpr RadixPlan z_radix_plan(int n) {
     for (int i = 0; i < fives; ++i) q.r[q.count++] = 5;
     return q;
 }
+
+// Alternative stage order of the z-type passes, timed by the autotuner
+// against z_radix_plan(n):
+//   powers of two: the radix-8 stages last and radix-4 stages first
+//     (1024 = 4*4*8*8, 256 = 4*8*8, 128 = 4*4*8, 32 = 4*8); a plan of three
+//     or more radix-8 stages only (512 = 8*8*8) instead takes its first two
+//     as 4*4*4 (512 = 4*4*4*8): one stage more, a radix-4 first stage;
+//   lengths with a factor 3 or 5: the default plan's last stage first
+//     (96 = 6*4*4, 48 = 3*4*4, 384 = 6*4*4*4, 80 = 5*4*4).
+// Where this gives the default plan again (64 = 8*8, 16 = 4*4, 8), the
+// length has no alternative (z_plan_has_alt).
+constexpr RadixPlan z_radix_plan_alt(int n) {
+    const RadixPlan p = z_radix_plan(n, 0);
+    if (p.count < 2) return p;
+    bool pow2 = true;
+    int eights = 0;
+    for (int s = 0; s < p.count; ++s) {
+        pow2 &= p.r[s] == 2 || p.r[s] == 4 || p.r[s] == 8;
+        eights += p.r[s] == 8;
+    }
+    RadixPlan q{};
+    if (pow2 && eights == p.count && eights >= 3) {
+        q.r[q.count++] = 4;
+        q.r[q.count++] = 4;
+        q.r[q.count++] = 4;
+        for (int s = 2; s < p.count; ++s) q.r[q.count++] = 8;
+    } else if (pow2) {
+        for (int s = 0; s < p.count; ++s)
+            if (p.r[s] != 8) q.r[q.count++] = p.r[s];
+        for (int s = 0; s < eights; ++s) q.r[q.count++] = 8;
+    } else {
+        q.r[q.count++] = p.r[p.count - 1];
+        for (int s = 0; s < p.count - 1; ++s) q.r[q.count++] = p.r[s];
+    }
+    return q;
+}
+
+constexpr bool z_plan_has_alt(int n) {
+    const RadixPlan a = z_radix_plan(n, 0), b = z_radix_plan(n, 1);
+    if (a.count != b.count) return true;
+    for (int s = 0; s < a.count; ++s)
+        if (a.r[s] != b.r[s]) return true;
+    return false;
+}
+
+// Template argument LINES of the z-type tiles and kernels: the tile width
+// (0 = the default) plus kOrderUnit times the radix order (0: z_radix_plan,
+// 1: z_radix_plan_alt).
+constexpr int kOrderUnit = 1024;
+constexpr int zlines(int lines, int order) { return lines + order * kOrderUnit; }
 
 // ---------------------------------------------------------------------------
 // Tile geometry per (storage type, length).
@@ -389,9 +444,11 @@ constexpr int ztile_stride(int nl) {
     return sizeof(T) == 4 && nl <= 8 ? nl : nl + 1;
 }
 
-template <typename T, int N, int NL_ = 0>  // NL_ = 0: the default width
+template <typename T, int N, int LINES_ = 0>  // zlines(width or 0 for the default, radix order)
 struct ZTile {
-    static constexpr RadixPlan P = z_radix_plan(N);
+    static constexpr int ORD = LINES_ / kOrderUnit;
+    static constexpr int NL_ = LINES_ % kOrderUnit;
+    static constexpr RadixPlan P = z_radix_plan(N, ORD);
     static constexpr int lcm_radix() {
         int l = 1;
         for (int s = 0; s < P.count; ++s) {
@@ -492,16 +549,17 @@ __device__ __forceinline__ void zstage_mid(const cx<T>* src, cx<T>* dst, const c
     __syncthreads();
 }
 
-// stages S .. count-2; NS is the product of the radices before stage S. The
-// data starts in a; with PP it alternates between a and b. Returns the tile
-// holding the result.
-template <typename T, int N, int NL, int NT, int NF, bool INV, int S, int NS, bool PP>
+// stages S .. count-2 of the plan of ZTile<T, N, LINES>; NS is the product
+// of the radices before stage S. The data starts in a; with PP it alternates
+// between a and b. Returns the tile holding the result.
+template <typename T, int N, int LINES, int NF, bool INV, int S, int NS, bool PP>
 __device__ __forceinline__ cx<T>* zstages_mid(cx<T>* a, cx<T>* b, const cx<T>* twl, int tid) {
-    constexpr RadixPlan P = z_radix_plan(N);
+    using Z = ZTile<T, N, LINES>;
+    constexpr RadixPlan P = Z::P;
     if constexpr (S <= P.count - 2) {
         cx<T>* const dst = PP ? b : a;
-        zstage_mid<T, N, NL, NT, NF, P.r[S], NS, INV, !PP>(a, dst, twl, tid);
-        return zstages_mid<T, N, NL, NT, NF, INV, S + 1, NS * P.r[S], PP>(dst, a, twl, tid);
+        zstage_mid<T, N, Z::NL, Z::NT, NF, P.r[S], NS, INV, !PP>(a, dst, twl, tid);
+        return zstages_mid<T, N, LINES, NF, INV, S + 1, NS * P.r[S], PP>(dst, a, twl, tid);
     } else {
         return a;
     }
@@ -509,9 +567,10 @@ __device__ __forceinline__ cx<T>* zstages_mid(cx<T>* a, cx<T>* b, const cx<T>* t
 
 // Middle stages of the z plan (ping-pong tiles alternate once per middle
 // stage; the first-stage write goes to tile A).
-template <int N>
+template <typename T, int N, int LINES = 0>
 constexpr int z_mid_stages() {
-    return z_radix_plan(N).count > 2 ? z_radix_plan(N).count - 2 : 0;
+    constexpr int c = ZTile<T, N, LINES>::P.count;
+    return c > 2 ? c - 2 : 0;
 }
 
 constexpr int kLdsBytes = 160 * 1024;  // LDS per work-group on gfx950
@@ -535,7 +594,7 @@ __device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>*
     // previous readers of tile A are done (ping-pong with an odd number of
     // middle stages: the last reads of A were the middle stage's, before
     // the barrier that ended it)
-    if constexpr (!PP || z_mid_stages<N>() % 2 == 0) __syncthreads();
+    if constexpr (!PP || z_mid_stages<T, N, LINES>() % 2 == 0) __syncthreads();
     // unpadded rows: lane group p = tid / NL writes element r ^ (p & SWM) in
     // write r
     constexpr bool kPow2 = (R0 & (R0 - 1)) == 0;
@@ -572,7 +631,7 @@ __device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>*
         }
     __syncthreads();
     // stages 1 .. count-2 (NS after stage 0 is R0)
-    return zstages_mid<T, N, NL, NT, NF, INV, 1, R0, PP>(lds, lds + NF * Z::LDS, twl, tid);
+    return zstages_mid<T, N, LINES, NF, INV, 1, R0, PP>(lds, lds + NF * Z::LDS, twl, tid);
 }
 
 // v: first-stage inputs (consumed), w: last-stage outputs. LINES: tile width

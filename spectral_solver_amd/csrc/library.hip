@@ -455,6 +455,30 @@ int sph_solver_tuned_pass(sph_solver_t solver, int pass, int* variant, int* line
     });
 }
 
+int sph_solver_radix_plan(sph_solver_t solver, int pass, int* radices, int* count) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(radices);
+        require_pointer(count);
+        const sph::RadixPlan p = solver->impl->radix_plan_of(pass);
+        for (int s = 0; s < p.count; ++s) radices[s] = p.r[s];
+        *count = p.count;
+    });
+}
+
+int sph_radix_plan(int n, int order, int* radices, int* count) {
+    return guarded([&] {
+        require_pointer(radices);
+        require_pointer(count);
+        if (order != 0 && order != 1) throw std::runtime_error("radix order must be 0 or 1");
+        *count = 0;
+        if (!sph::pass_table<float>(n)) return;
+        const sph::RadixPlan p = sph::z_radix_plan(n, order);
+        for (int s = 0; s < p.count; ++s) radices[s] = p.r[s];
+        *count = p.count;
+    });
+}
+
 int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms) {
     return guarded([&] {
         require_handle(solver);

@@ -106,7 +106,7 @@ __device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
 // kernels already run one 1024-thread work-group per CU there).
 template <typename T, int N, int LINES = 0>
 constexpr bool z_pingpong() {
-    return N >= 256 && z_mid_stages<N>() >= 1 &&
+    return N >= 256 && z_mid_stages<T, N, LINES>() >= 1 &&
            (2 * ZTile<T, N, LINES>::LDS + N) * static_cast<int>(sizeof(cx<T>)) <= kLdsBytes;
 }
 template <typename T, int N, int LINES = 0>

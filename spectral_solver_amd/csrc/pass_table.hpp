@@ -14,12 +14,20 @@ namespace sph {
 //             it (512^3 FP32 keeps the quarter: more, smaller work-groups per
 //             CU hide the latency of the tile loads better);
 //   x pass:   register-direct tiles of the default z-tile width, then half of
-//             it, then the LDS-staged form at the default width;
-//   forward z: the default z-tile width and half of it (half: twice the
-//             work-groups per CU where they fit; the standalone inverse z
-//             takes the same width).
+//             it, then the LDS-staged form at the default width, then
+//             register-direct tiles of the default width in the alternative
+//             radix order (z_radix_plan_alt; lengths that have one);
+//   forward z: the default z-tile width, half of it (half: twice the
+//             work-groups per CU where they fit), then the default width in
+//             the alternative radix order (the standalone inverse z takes the
+//             same width and order).
 constexpr int kTunedPasses = 4;
-constexpr int kMaxVariants = 3;
+constexpr int kMaxVariants = 4;
+// the x-pass variant of the LDS-staged form and of the alternative radix
+// order, the forward-z variant of the latter
+constexpr int kXLdsVariant = 2;
+constexpr int kXOrderVariant = 3;
+constexpr int kZOrderVariant = 2;
 
 template <typename T>
 struct PassTable {
@@ -28,6 +36,9 @@ struct PassTable {
     int nt;  // threads per work-group
     int variants[kTunedPasses];                // variants of each tuned pass
     int tile_lines[kTunedPasses][kMaxVariants];  // lines per tile of each variant
+    // radix order of each variant (0: z_radix_plan, 1: z_radix_plan_alt; the
+    // y passes always 0)
+    int radix_order[kTunedPasses][kMaxVariants];
     void (*inv_z)(const InvZArgs<T>&, int nky, int nkx, hipStream_t);
     void (*lines)(cx<T>* data, long long nlines, bool inverse, const cx<T>* tw, hipStream_t);
     void (*inv_x)(const InvXArgs<T>&, int ny, int nz, hipStream_t);
@@ -39,10 +50,11 @@ struct PassTable {
     void (*plines)(cx<T>* data, long long npairlines, long long member_stride, int members, bool inverse,
                    const cx<T>* tw, int variant, hipStream_t);
     void (*pinv_x)(const PairInvXArgs<T>&, int ny, int nz, int members, hipStream_t);
-    // xt: the transposed work layout (register-direct variants 0 and 1 only)
+    // xt: the transposed work layout (register-direct variants 0, 1 and
+    // kXOrderVariant only)
     void (*px_products)(const PairXArgs<T>&, int ny, int nz, int members, int variant, bool xt, hipStream_t);
     // the register-direct x pass of one cubic solve that also reduces max
-    // |u|^2 into a.vmax2 (variants 0 and 1; px_products_vmax)
+    // |u|^2 into a.vmax2 (variants 0, 1 and kXOrderVariant; px_products_vmax)
     void (*px_products_vmax)(const PairXArgs<T>&, int variant, bool xt, hipStream_t);
     // forward z + update, then the next stage's inverse z (k_pfwd_z_rk_inv)
     void (*pfwd_z_rk)(const PairFwdZArgs<T>&, const PairInvZArgs<T>&, int nky, int nkx, int members, int variant,
@@ -115,6 +127,10 @@ struct Launch {
     // whole number of wavefronts
     static constexpr int Z0 = ZT::NL;
     static constexpr int Z1 = Z0 / 2 >= 2 && (Z0 / 2) * N / ZT::E >= 64 ? Z0 / 2 : Z0;
+    // the alternative radix order at its own default width
+    static constexpr bool kAlt = z_plan_has_alt(N);
+    static constexpr int ZA = zlines(0, 1);
+    static constexpr int ZA_LINES = ZTile<T, N, ZA>::NL;
     template <int L>
     static void pinv_z_l(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
@@ -130,6 +146,8 @@ struct Launch {
                        hipStream_t s) {
         if (variant == 1 && mode == PZ_RHS)
             pinv_z_l<Z1>(a, mode, nky, nkx, members, s);
+        else if (kAlt && variant == kZOrderVariant && mode == PZ_RHS)
+            pinv_z_l<ZA>(a, mode, nky, nkx, members, s);
         else
             pinv_z_l<Z0>(a, mode, nky, nkx, members, s);
     }
@@ -166,7 +184,7 @@ struct Launch {
     template <int L, bool XT = false>
     static void px_products_rd(const PairXArgs<T>& a, int ny, int nz, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
-        const dim3 grid((ny + L - 1) / L, nz, members);
+        const dim3 grid((ny + ZV::NL - 1) / ZV::NL, nz, members);
         k_px_products_rd<T, N, L, false, XT><<<grid, ZV::NT, 0, s>>>(a);
     }
     // the transposed work layout (k_plines_t, k_px_products_rd<..., XT>) at
@@ -208,6 +226,8 @@ struct Launch {
             if (xt) {
                 if (variant == 1)
                     px_products_rd<X1, true>(a, ny, nz, members, s);
+                else if (kAlt && variant == kXOrderVariant)
+                    px_products_rd<ZA, true>(a, ny, nz, members, s);
                 else
                     px_products_rd<X0, true>(a, ny, nz, members, s);
                 return;
@@ -215,11 +235,13 @@ struct Launch {
         }
         // the register-direct form addresses rows of at most N * N modes
         if (static_cast<long long>(ny) * nz > static_cast<long long>(N) * N) variant = 2;
-        if (variant == 2) {
+        if (variant == kXLdsVariant) {
             const dim3 grid((ny + L0 - 1) / L0, nz, members);
             k_px_products<T, N, L0><<<grid, TL::NT, 0, s>>>(a);
         } else if (variant == 1) {
             px_products_rd<X1>(a, ny, nz, members, s);
+        } else if (kAlt && variant == kXOrderVariant) {
+            px_products_rd<ZA>(a, ny, nz, members, s);
         } else {
             px_products_rd<X0>(a, ny, nz, members, s);
         }
@@ -235,14 +257,19 @@ struct Launch {
                           int variant, hipStream_t s) {
         if (variant == 1)
             pfwd_z_rk_l<Z1>(a, inv, nky, nkx, members, s);
+        else if (kAlt && variant == kZOrderVariant)
+            pfwd_z_rk_l<ZA>(a, inv, nky, nkx, members, s);
         else
             pfwd_z_rk_l<Z0>(a, inv, nky, nkx, members, s);
     }
     static constexpr PassTable<T> table{N,
                                         TL::NL,
                                         TL::NT,
-                                        {3, 3, 3, Z1 != Z0 ? 2 : 1},
-                                        {{L0, L1, L2}, {X0, X1, L0}, {L0, L1, L2}, {Z0, Z1, Z1}},
+                                        {3, kAlt ? 4 : 3, 3, kAlt ? 3 : (Z1 != Z0 ? 2 : 1)},
+                                        {{L0, L1, L2, L2}, {X0, X1, L0, ZA_LINES}, {L0, L1, L2, L2},
+                                         {Z0, Z1, ZA_LINES, ZA_LINES}},
+                                        {{0, 0, 0, 0}, {0, 0, 0, kAlt ? 1 : 0}, {0, 0, 0, 0},
+                                         {0, 0, kAlt ? 1 : 0, kAlt ? 1 : 0}},
                                         &inv_z,
                                         &lines,
                                         &inv_x,
@@ -255,7 +282,7 @@ struct Launch {
                                         &px_products_vmax<T, N>,
                                         &pfwd_z_rk,
                                         kXT,
-                                        {T0L, T1L, T2L},
+                                        {T0L, T1L, T2L, T2L},
                                         &plines_t};
 };
 

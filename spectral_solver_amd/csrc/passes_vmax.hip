@@ -6,24 +6,34 @@
 
 namespace sph {
 
+template <typename T, int N, int L, bool XT>
+void px_products_vmax_l(const PairXArgs<T>& a, hipStream_t s) {
+    using ZV = ZTile<T, N, L>;
+    const dim3 grid((a.ny + ZV::NL - 1) / ZV::NL, a.nz, 1);
+    k_px_products_rd<T, N, L, true, XT><<<grid, ZV::NT, 0, s>>>(a);
+}
+
 template <typename T, int N>
 void px_products_vmax(const PairXArgs<T>& a, int variant, bool xt, hipStream_t s) {
-    constexpr int L = Launch<T, N>::X0;
-    constexpr int L1 = Launch<T, N>::X1;
-    const dim3 grid((a.ny + (variant == 1 ? L1 : L) - 1) / (variant == 1 ? L1 : L), a.nz, 1);
-    if constexpr (Launch<T, N>::kXT) {
+    using LN = Launch<T, N>;
+    constexpr int L = LN::X0, L1 = LN::X1, LA = LN::ZA;
+    if constexpr (LN::kXT) {
         if (xt) {
             if (variant == 1)
-                k_px_products_rd<T, N, L1, true, true><<<grid, ZTile<T, N, L1>::NT, 0, s>>>(a);
+                px_products_vmax_l<T, N, L1, true>(a, s);
+            else if (LN::kAlt && variant == kXOrderVariant)
+                px_products_vmax_l<T, N, LA, true>(a, s);
             else
-                k_px_products_rd<T, N, L, true, true><<<grid, ZTile<T, N, L>::NT, 0, s>>>(a);
+                px_products_vmax_l<T, N, L, true>(a, s);
             return;
         }
     }
     if (variant == 1)
-        k_px_products_rd<T, N, L1, true><<<grid, ZTile<T, N, L1>::NT, 0, s>>>(a);
+        px_products_vmax_l<T, N, L1, false>(a, s);
+    else if (LN::kAlt && variant == kXOrderVariant)
+        px_products_vmax_l<T, N, LA, false>(a, s);
     else
-        k_px_products_rd<T, N, L, true><<<grid, ZTile<T, N, L>::NT, 0, s>>>(a);
+        px_products_vmax_l<T, N, L, false>(a, s);
 }
 
 #define SPH_VMAX(N)                                                                                 \

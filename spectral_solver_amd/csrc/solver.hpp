@@ -205,6 +205,15 @@ public:
         *lines = xt && p != 1 ? pt_->xt_lines[v] : pt_->tile_lines[p][v];
         for (int q = 0; q < kMaxVariants; ++q) ms[q] = xt ? xtune_ms_[p][q] : tune_ms_[p][q];
     }
+    // the radix stages of the kept variant of tuned pass p (the y passes and
+    // the LDS-staged x pass use radix_plan, the z-type passes their order)
+    RadixPlan radix_plan_of(int p) const {
+        if (p < 0 || p >= kTuned) throw std::runtime_error("tuned pass out of range");
+        const bool xt = xt_ && p < 3;
+        const int v = xt ? xvar_[p] : var_[p];
+        if (p == 0 || p == 2 || (p == 1 && !xt && v == kXLdsVariant)) return radix_plan(n);
+        return z_radix_plan(n, pt_->radix_order[p][v]);
+    }
     // the work layout kept (1: transposed between the y passes and the x
     // pass) and the summed time of passes 1-3 with each layout's fastest
     // variants (0 where a layout was not timed)
@@ -376,7 +385,7 @@ public:
     // dt either (launch_step_mid); the tail is the rest of the step, from
     // stage 0's forward z and update on. One solve per group, register-direct
     // x pass only.
-    bool x_reduces_velocity() const { return count == 1 && (xt_ || var_[1] != 2); }
+    bool x_reduces_velocity() const { return count == 1 && (xt_ || var_[1] != kXLdsVariant); }
     void launch_step_head(unsigned long long* vmax2, hipStream_t s) const {
         k_zero_vmax<<<1, kVmaxSlots, 0, s>>>(vmax2);
         launch_pass(1, 0, U_.get(), G_.get(), 0, 1, s);
@@ -600,9 +609,13 @@ private:
         const bool xt_allowed = pt_->xt && forced_layout != 0 && transposed_fits();
         const char* force = std::getenv("SPH_TILE_VARIANT");
         if (force && force[0] >= '0' && force[0] < '0' + kMaxVariants && force[1] == '\0') {
+            const int f = force[0] - '0';
             for (int p = 0; p < kTuned; ++p) {
-                var_[p] = std::min(force[0] - '0', pt_->variants[p] - 1);
-                xvar_[p] = p == 3 ? 0 : std::min(force[0] - '0', p == 1 ? 1 : kMaxVariants - 1);
+                var_[p] = std::min(f, pt_->variants[p] - 1);
+                // transposed layout: y passes 0..2; x pass 0, 1 and the
+                // alternative radix order (no LDS-staged form)
+                const bool x_alt = f == kXOrderVariant && pt_->variants[1] > kXOrderVariant;
+                xvar_[p] = p == 3 ? 0 : (p == 1 ? (x_alt ? f : std::min(f, 1)) : std::min(f, 2));
             }
             use_transposed(forced_layout == 1 && xt_allowed);
             return;
@@ -643,12 +656,15 @@ private:
         // every variant of the tuned passes in the current layout (nvar[p]
         // of pass p; 0: not tuned here); returns the summed time of the
         // fastest of passes 1-3, the ones the layouts differ in
-        auto tune_layout = [&](int (&var)[kTuned], double (&ms)[kTuned][kMaxVariants], const int* nvar) {
+        auto tune_layout = [&](int (&var)[kTuned], double (&ms)[kTuned][kMaxVariants], const int* nvar,
+                               bool transposed) {
             double total = 0.0;
             for (int p = 0; p < kTuned; ++p) {
                 if (nvar[p] < 1) continue;
                 int best = 0;
                 for (int v = 0; v < nvar[p]; ++v) {
+                    // the transposed layout has no LDS-staged x pass
+                    if (transposed && p == 1 && v == kXLdsVariant) continue;
                     var[p] = v;
                     launch_pass(pass_of[p], 0, U_.get(), G_.get(), 0, count, s);
                     hip_check(hipEventRecord(a, s), "hipEventRecord");
@@ -666,12 +682,12 @@ private:
             return total;
         };
         use_transposed(false);
-        layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants);
+        layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants, false);
         bool keep_xt = false;
         if (xt_allowed) {
             use_transposed(true);
-            const int nx[kTuned] = {kMaxVariants, 2, kMaxVariants, 0};
-            layout_ms_[1] = tune_layout(xvar_, xtune_ms_, nx);
+            const int nx[kTuned] = {3, pt_->variants[1], 3, 0};
+            layout_ms_[1] = tune_layout(xvar_, xtune_ms_, nx, true);
             keep_xt = forced_layout == 1 || layout_ms_[1] < layout_ms_[0];
         }
         use_transposed(keep_xt);
@@ -828,6 +844,7 @@ public:
     virtual bool fused_velocity_max() const = 0;
     virtual void tuning(int* variant, int* lines, double* ms) const = 0;
     virtual void tuned_pass(int pass, int* variant, int* lines, double* ms) const = 0;
+    virtual RadixPlan radix_plan_of(int pass) const = 0;
     virtual void layout(int* transposed, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
@@ -995,6 +1012,7 @@ public:
     void tuned_pass(int pass, int* variant, int* lines, double* ms) const override {
         g_.tuned_pass(pass, variant, lines, ms);
     }
+    RadixPlan radix_plan_of(int pass) const override { return g_.radix_plan_of(pass); }
     void layout(int* transposed, double* ms) const override { g_.layout(transposed, ms); }
 
     // One adaptive step as the driver takes it (impl.hpp:1363-1378, without

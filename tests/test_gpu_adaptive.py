@@ -3,8 +3,9 @@ velocity maximum of the state it transforms, passes_vmax.hip) on every x-pass
 form, in FP64 and FP32:
 
   - the GPU driver's CSV against the oracle driver's for a short adaptive-CFL
-    run with each autotuner variant forced (SPH_TILE_VARIANT 0 and 1 run the
-    fused reduction, 2 the LDS-staged x pass and max_velocity() instead);
+    run with each autotuner variant forced (SPH_TILE_VARIANT 0, 1 and 3 -- the
+    alternative radix order -- run the fused reduction, 2 the LDS-staged x
+    pass and max_velocity() instead);
   - sph_solver_step_adaptive: its max |u| equals max_velocity() of the same
     state and its dt is min(dt_max, cfl h / max|u|);
   - an FP32 adaptive run: every step's main_dt column follows from the
@@ -27,7 +28,7 @@ def read_csv_rows(path):
     return lines[0], np.array([[float(v) for v in ln.split(",")] for ln in lines[1:]])
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3"])
 def test_adaptive_driver_matches_oracle_for_every_x_form(tmp_path, monkeypatch, variant):
     print("case adaptive driver, x-pass variant", variant, flush=True)
     monkeypatch.setenv("SPH_TILE_VARIANT", variant)
@@ -52,8 +53,9 @@ def test_adaptive_driver_matches_oracle_for_every_x_form(tmp_path, monkeypatch, 
 
 
 @pytest.mark.parametrize("precision,n,variant,tol", [("fp64", 32, "0", 1e-12), ("fp64", 64, "1", 1e-12),
-                                                     ("fp32", 64, "0", 2e-6), ("fp32", 64, "1", 2e-6),
-                                                     ("fp32", 128, "0", 2e-6)])
+                                                     ("fp64", 32, "3", 1e-12), ("fp32", 64, "0", 2e-6),
+                                                     ("fp32", 64, "1", 2e-6), ("fp32", 128, "0", 2e-6),
+                                                     ("fp32", 128, "3", 2e-6)])
 def test_step_adaptive_velocity_max_equals_max_velocity(monkeypatch, precision, n, variant, tol):
     print("case step_adaptive", precision, n, "x-pass variant", variant, flush=True)
     monkeypatch.setenv("SPH_TILE_VARIANT", variant)

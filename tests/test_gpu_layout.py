@@ -34,7 +34,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("n,precision,variant,tol", [(256, "fp64", None, 1e-14), (256, "fp64", "1", 1e-14),
-                                                     (256, "fp64", "2", 1e-14), (512, "fp32", None, 1e-6),
+                                                     (256, "fp64", "2", 1e-14), (256, "fp64", "3", 1e-14),
+                                                     (512, "fp32", None, 1e-6),
                                                      (256, "fp32", "1", 1e-6)])
 def test_transposed_layout_agrees_with_line_layout(monkeypatch, n, precision, variant, tol):
     print("case layout", n, precision, "variant", variant, flush=True)

@@ -266,8 +266,9 @@ def test_cli_fixed_dt(tmp_path):
 
 @pytest.mark.parametrize("precision,n", [("fp64", 32), ("fp32", 64), ("fp64", 128)])
 def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
-    """Every autotuner variant of the y and x passes (tile widths, and the
-    register-direct and LDS-staged x pass) gives the oracle's state (FP64:
+    """Every autotuner variant of the y and x passes (tile widths, the
+    register-direct and LDS-staged x pass, the alternative radix order of the
+    register-direct x pass and the forward-z pass) gives the oracle's state (FP64:
     1e-12; FP32: the FP32 form's rounding) and the others' to rounding (the
     compiler contracts multiply-adds differently in different kernels, so the
     bits may differ). In FP32 at 64^3 the 8- and 4-line y tiles use the
@@ -277,17 +278,28 @@ def test_tile_variants_agree_and_match_oracle(monkeypatch, precision, n):
     steps = 3
     o, u = random_state(n, 11)
     states = []
-    for v in ("0", "1", "2"):
+    alt = sph.radix_plan(n, 1) != sph.radix_plan(n, 0)  # 32 and 128: yes, 64: no
+    for v in ("0", "1", "2", "3"):
         print("case tile variant", precision, n, v, flush=True)
         monkeypatch.setenv("SPH_TILE_VARIANT", v)
         g = sph.TaylorGreenSolver(n, 400.0, precision)
         k = int(v)
         tv = g.tuning()
-        assert [tv[p][0] for p in ("inverse_y", "x_products", "forward_y")] == [k, k, k]
-        # the forward-z pass has two widths where the half tile still fills
-        # whole wavefronts (64^3 FP32: 16 and 8 lines; 128^3 FP64: 8 and 4),
-        # else one
-        assert tv["forward_z_update"][0] == (min(k, 1) if (precision, n) in (("fp32", 64), ("fp64", 128)) else 0)
+        # y passes: three widths; x pass: variant 3 (the alternative radix
+        # order) where the length has one, else the last variant (LDS-staged)
+        assert [tv[p][0] for p in ("inverse_y", "x_products", "forward_y")] == \
+            [min(k, 2), k if (k < 3 or alt) else 2, min(k, 2)]
+        # the forward-z pass: the default width, half of it where the half
+        # tile still fills whole wavefronts (64^3 FP32: 16 and 8 lines;
+        # 128^3 FP64: 8 and 4; otherwise variant 1 is the default width
+        # again), and the alternative radix order where there is one
+        nz = 3 if alt else (2 if (precision, n) in (("fp32", 64), ("fp64", 128)) else 1)
+        assert tv["forward_z_update"][0] == min(k, nz - 1)
+        plans = g.radix_plans()
+        want_x = sph.radix_plan(n, 1) if (k == 3 and alt) else (None if k == 2 or k == 3 else sph.radix_plan(n, 0))
+        if want_x is not None:
+            assert plans["x_products"] == want_x
+        assert plans["forward_z_update"] == sph.radix_plan(n, 1 if (alt and k >= 2) else 0)
         for c in range(3):
             g.set_u_hat(c, oracle.c2c_to_half(u[c]))
         for _ in range(steps):
