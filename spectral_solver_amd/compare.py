@@ -26,34 +26,33 @@ import numpy as np
 QUANTITIES = ("kinetic_energy", "dissipation", "enstrophy")
 
 
+DAT_COLUMNS = ("t",) + QUANTITIES  # column order of the whitespace table
+CSV_REQUIRED = ("t", "kinetic_energy", "dissipation")
+
+
 def load_reference(path) -> dict:
-    """Whitespace table, at least four columns t E epsilon enstrophy
-    (compare_taylor_green.py:35-44)."""
-    data = np.loadtxt(path, ndmin=2)
-    if data.ndim != 2 or data.shape[1] < 4:
-        raise ValueError(f"{path} must have at least four columns: t E epsilon enstrophy")
-    return {
-        "t": data[:, 0],
-        "kinetic_energy": data[:, 1],
-        "dissipation": data[:, 2],
-        "enstrophy": data[:, 3],
-    }
+    """Whitespace table whose first four columns are t, E, epsilon and
+    enstrophy (compare_taylor_green.py:35-44); further columns are ignored."""
+    table = np.loadtxt(path, ndmin=2)
+    if table.shape[1] < len(DAT_COLUMNS):
+        raise ValueError(f"{path}: {table.shape[1]} columns per row, the table needs four "
+                         f"({' '.join(DAT_COLUMNS)})")
+    return {name: table[:, i].copy() for i, name in enumerate(DAT_COLUMNS)}
 
 
 def load_ours(path, reynolds: float) -> dict:
     """The solver's CSV (header row of names). t, kinetic_energy and
     dissipation are required; enstrophy falls back to dissipation / (2 nu)
     with nu = 1 / reynolds (compare_taylor_green.py:47-65)."""
-    data = np.genfromtxt(path, delimiter=",", names=True, ndmin=1)
-    names = set(data.dtype.names or ())
-    missing = {"t", "kinetic_energy", "dissipation"} - names
-    if missing:
-        raise ValueError(f"{path} is missing required columns: {', '.join(sorted(missing))}")
-    out = {k: np.atleast_1d(np.asarray(data[k], dtype=np.float64)) for k in ("t", "kinetic_energy", "dissipation")}
-    if "enstrophy" in names:
-        out["enstrophy"] = np.atleast_1d(np.asarray(data["enstrophy"], dtype=np.float64))
-    else:
-        out["enstrophy"] = out["dissipation"] / (2.0 * (1.0 / reynolds))
+    rec = np.genfromtxt(path, delimiter=",", names=True, ndmin=1)
+    have = rec.dtype.names or ()
+    absent = [name for name in CSV_REQUIRED if name not in have]
+    if absent:
+        raise ValueError(f"{path}: header lacks {' and '.join(absent)}")
+    col = lambda name: np.atleast_1d(np.asarray(rec[name], dtype=np.float64))
+    out = {name: col(name) for name in CSV_REQUIRED}
+    # dissipation = 2 nu enstrophy in the solver's output
+    out["enstrophy"] = col("enstrophy") if "enstrophy" in have else out["dissipation"] * (0.5 * reynolds)
     return out
 
 

@@ -35,7 +35,7 @@ def test_reference_loader_reads_the_literature_curve():
 def test_reference_loader_rejects_narrow_tables(tmp_path):
     p = tmp_path / "bad.dat"
     p.write_text("0 1 2\n1 2 3\n")
-    with pytest.raises(ValueError, match="four columns"):
+    with pytest.raises(ValueError, match="needs four"):
         compare.load_reference(p)
 
 
