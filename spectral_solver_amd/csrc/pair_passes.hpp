@@ -81,19 +81,30 @@ struct alignas(4 * sizeof(T)) cx2 {
 
 // cx2 through a raw buffer resource (32-bit offset; reads past the resource's
 // extent return zeros)
-template <typename T>
+template <typename T, int POL = kPolicyDefault>
 __device__ __forceinline__ cx2<T> ld_cx2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     using V4 = int __attribute__((ext_vector_type(4)));
     cx2<T> q;
     if constexpr (sizeof(T) == 4) {
-        q = __builtin_bit_cast(cx2<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        q = __builtin_bit_cast(cx2<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL));
     } else {
-        const V4 lo = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-        const V4 hi = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0);
+        const V4 lo = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL);
+        const V4 hi = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, POL);
         q.a = __builtin_bit_cast(cx<T>, lo);
         q.b = __builtin_bit_cast(cx<T>, hi);
     }
     return q;
+}
+
+// store of a cx2 with cache policy POL
+template <int POL, typename T>
+__device__ __forceinline__ void gst2(cx2<T>* p, cx2<T> v) {
+    if constexpr (POL == kPolicyStream) {
+        using V = T __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(V{v.a.x, v.a.y, v.b.x, v.b.y}, reinterpret_cast<V*>(p));
+    } else {
+        *p = v;
+    }
 }
 
 template <typename T>
@@ -289,7 +300,8 @@ __device__ __forceinline__ void plines_tile(cx<T>* data, long long npairlines, l
     for (int i = 0; i < E; ++i) {
         const int m = tid + i * NT;
         const int g = m / (2 * N), rem = m % (2 * N);
-        if (m < TL::ELEMS) buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = m < valid ? base[m] : czero<T>();
+        if (m < TL::ELEMS)
+            buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = m < valid ? gld<kPolicyStream>(base + m) : czero<T>();
     }
     __syncthreads();
     fft_tile<T, N, NL, NT, INV, 0, 1, SWZ>(buf, twl, tid);
@@ -297,7 +309,7 @@ __device__ __forceinline__ void plines_tile(cx<T>* data, long long npairlines, l
     for (int i = 0; i < E; ++i) {
         const int m = tid + i * NT;
         const int g = m / (2 * N), rem = m % (2 * N);
-        if (m < valid) base[m] = buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))];
+        if (m < valid) gst<kPolicyStream>(base + m, buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
     }
 }
 
@@ -362,10 +374,10 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
         if (TL::ELEMS % NT == 0 || m < TL::ELEMS) {
             if constexpr (INV) {
                 const int g = m / (2 * N), rem = m % (2 * N);  // pair line g, element rem >> 1, member rem & 1
-                buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = 2 * g < lines ? src[lin_off(g, rem)] : czero<T>();
+                buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = 2 * g < lines ? gld<kPolicyDefault>(src + lin_off(g, rem)) : czero<T>();
             } else {
                 const int y = m / NL, c = m % NL;  // row y, value c = 2g + f
-                buf[tile_at<NL, SWZ>(y, c)] = c < lines ? src[tr_off(y, c)] : czero<T>();
+                buf[tile_at<NL, SWZ>(y, c)] = c < lines ? gld<kPolicyDefault>(src + tr_off(y, c)) : czero<T>();
             }
         }
     }
@@ -377,10 +389,10 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
         if (TL::ELEMS % NT == 0 || m < TL::ELEMS) {
             if constexpr (INV) {
                 const int y = m / NL, c = m % NL;
-                if (c < lines) dst[tr_off(y, c)] = buf[tile_at<NL, SWZ>(y, c)];
+                if (c < lines) gst<kPolicyDefault>(dst + tr_off(y, c), buf[tile_at<NL, SWZ>(y, c)]);
             } else {
                 const int g = m / (2 * N), rem = m % (2 * N);
-                if (2 * g < lines) dst[lin_off(g, rem)] = buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))];
+                if (2 * g < lines) gst<kPolicyDefault>(dst + lin_off(g, rem), buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
             }
         }
     }
@@ -458,7 +470,7 @@ __device__ __forceinline__ void products_forward_x(const T (&u)[3][E], const T (
                     cx2<T> q;
                     q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                     q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                    reinterpret_cast<cx2<T>*>(o)[off] = q;
+                    gst2<kPolicyDefault>(reinterpret_cast<cx2<T>*>(o) + off, q);
                 }
             }
         }
@@ -575,6 +587,10 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     // wavefront and the pair separation is a lane exchange instead of a trip
     // through an LDS tile with two barriers
     constexpr bool MIRL = Z::kMirrorableLast;
+    // cache policy of the pair rows: NT in FP32 in the line layout (each row
+    // segment is 128 bytes of one tile; 512^3: 2.50 -> 2.40 ms), the default
+    // in FP64 (5.6 -> 6.2 ms with NT) and in the transposed layout
+    constexpr int XPOL = sizeof(T) == 4 && !XT ? kPolicyStream : kPolicyDefault;
     const int l = tid % NL;
     const int y0 = tile.x * NL;
     const int z = tile.y;
@@ -625,14 +641,14 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 cx2<T> q;
                 if constexpr (XT) {
                     const uint32_t off = ok ? (l * H + k) * static_cast<uint32_t>(sizeof(cx2<T>)) : 0xFFFFFFC0u;
-                    q = ld_cx2<T>(rsx, off);
+                    q = ld_cx2<T, XPOL>(rsx, off);
                 } else if constexpr (kNarrow) {
                     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                         const_cast<cx2<T>*>(AB + static_cast<size_t>(kmin) * row), 0,
                         static_cast<int>(STEP * row * sizeof(cx2<T>)), 0x00020000);
                     const uint32_t off =
                         ok ? ((k - kmin) * row + l) * static_cast<uint32_t>(sizeof(cx2<T>)) : 0xFFFFFFC0u;
-                    q = ld_cx2<T>(rs, off);
+                    q = ld_cx2<T, XPOL>(rs, off);
                 } else {
                     q = AB[static_cast<size_t>(k) * row + (ok ? l : 0)];
                     if (!ok) q = cx2<T>{};
@@ -713,7 +729,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 cx2<T> q;
                 q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                 q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-                if (ok && (r < HR || self0)) o[static_cast<size_t>(j + r * Z::STEP) * ostep] = q;
+                if (ok && (r < HR || self0)) gst2<XPOL>(o + static_cast<size_t>(j + r * Z::STEP) * ostep, q);
             }
             continue;
         }
@@ -736,9 +752,9 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                     q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                     q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
                     if constexpr (XT)
-                        o[xt_block + static_cast<size_t>(l2) * H + k] = q;
+                        gst2<XPOL>(o + xt_block + static_cast<size_t>(l2) * H + k, q);
                     else
-                        o[k * row + static_cast<size_t>(z) * a.ny + y0 + l2] = q;
+                        gst2<XPOL>(o + k * row + static_cast<size_t>(z) * a.ny + y0 + l2, q);
                 }
             }
         }

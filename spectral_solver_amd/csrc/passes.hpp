@@ -44,28 +44,60 @@ __device__ __forceinline__ bool keep_mode(int k2, int n) {
 // 32-bit byte offset per lane, so one offset register per slot serves every
 // array of the same layout. Accesses beyond `bytes` read 0 / are dropped.
 // Resource word 3 for gfx9-family raw buffers: 0x00020000.
-template <typename T>
+// Cache policy of global loads and stores (the aux operand of the buffer
+// and global instructions): bit 1 is gfx950's non-temporal hint, NT. The y
+// passes stream whole contiguous tiles that no other work-group touches and
+// store them with NT (512^3 FP32: inverse y 1.21 -> 1.14 ms, forward y 2.38 ->
+// 2.26 ms); the z passes keep the default (their 24/48-byte state rows share
+// 128-byte lines with the neighbouring tile, which the L2 serves: with NT the
+// forward-z pass took 3.8 instead of 3.4 ms), and so do accesses of partial
+// 128-byte lines (the transposed layout's rows).
+constexpr int kPolicyDefault = 0;
+constexpr int kPolicyStream = 2;
+
+template <typename T, int POL = kPolicyDefault>
 struct Buf {
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ Buf(const cx<T>* base, uint32_t bytes)
         : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<cx<T>*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
     __device__ __forceinline__ cx<T> ld(uint32_t off) const {
         if constexpr (sizeof(T) == 4) {
-            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, POL));
         } else {
-            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+            return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL));
         }
     }
     __device__ __forceinline__ void st(uint32_t off, cx<T> v) const {
         if constexpr (sizeof(T) == 4) {
             using V = int __attribute__((ext_vector_type(2)));
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, off, 0, POL);
         } else {
             using V = int __attribute__((ext_vector_type(4)));
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, off, 0, POL);
         }
     }
 };
+
+// Plain-pointer accesses with cache policy POL.
+template <int POL, typename T>
+__device__ __forceinline__ cx<T> gld(const cx<T>* p) {
+    if constexpr (POL == kPolicyStream) {
+        using V = T __attribute__((ext_vector_type(2)));
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+        return {v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <int POL, typename T>
+__device__ __forceinline__ void gst(cx<T>* p, cx<T> v) {
+    if constexpr (POL == kPolicyStream) {
+        using V = T __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(V{v.x, v.y}, reinterpret_cast<V*>(p));
+    } else {
+        *p = v;
+    }
+}
 
 // Work-groups are dealt to the 8 XCDs round-robin by linear id, and each XCD
 // has its own L2. Renumber them so that consecutive logical tiles (x fastest)
