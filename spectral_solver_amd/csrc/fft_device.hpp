@@ -507,10 +507,26 @@ struct ZTile {
     __host__ __device__ static int e_last_mirror(int tid, int r) { return j_mirror(tid) + r * (N / RL); }
 };
 
+// Work-group barrier of the z transforms. LB: for LDS only -- this wave's
+// LDS accesses complete, then the hardware barrier -- so that global-to-LDS
+// loads of another buffer (a prefetch) stay in flight across it; a
+// __syncthreads() would also wait for them (its release fence counts them).
+template <bool LB>
+__device__ __forceinline__ void zbarrier() {
+    if constexpr (LB) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left at their maxima
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    } else {
+        __syncthreads();
+    }
+}
+
 // One middle Stockham stage over NF field tiles. In place (src == dst): read
 // all, sync, write all, sync. Ping-pong (src != dst): read src, write dst,
 // sync; the reads of src and writes of dst need no barrier between them.
-template <typename T, int N, int NL, int NT, int NF, int R, int NS, bool INV, bool INPLACE>
+template <typename T, int N, int NL, int NT, int NF, int R, int NS, bool INV, bool INPLACE, bool LB = false>
 __device__ __forceinline__ void zstage_mid(const cx<T>* src, cx<T>* dst, const cx<T>* twl, int tid) {
     constexpr int M = N / R;
     constexpr int TASKS = NL * M;
@@ -535,7 +551,7 @@ __device__ __forceinline__ void zstage_mid(const cx<T>* src, cx<T>* dst, const c
             dft<R, INV>(v[f][i]);
         }
     }
-    if constexpr (INPLACE) __syncthreads();
+    if constexpr (INPLACE) zbarrier<LB>();
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
         const int t = tid + i * NT;
@@ -546,20 +562,20 @@ __device__ __forceinline__ void zstage_mid(const cx<T>* src, cx<T>* dst, const c
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[f * (S * N) + (base + r * NS) * S + line] = v[f][i][r];
     }
-    __syncthreads();
+    zbarrier<LB>();
 }
 
 // stages S .. count-2 of the plan of ZTile<T, N, LINES>; NS is the product
 // of the radices before stage S. The data starts in a; with PP it alternates
 // between a and b. Returns the tile holding the result.
-template <typename T, int N, int LINES, int NF, bool INV, int S, int NS, bool PP>
+template <typename T, int N, int LINES, int NF, bool INV, int S, int NS, bool PP, bool LB = false>
 __device__ __forceinline__ cx<T>* zstages_mid(cx<T>* a, cx<T>* b, const cx<T>* twl, int tid) {
     using Z = ZTile<T, N, LINES>;
     constexpr RadixPlan P = Z::P;
     if constexpr (S <= P.count - 2) {
         cx<T>* const dst = PP ? b : a;
-        zstage_mid<T, N, Z::NL, Z::NT, NF, P.r[S], NS, INV, !PP>(a, dst, twl, tid);
-        return zstages_mid<T, N, LINES, NF, INV, S + 1, NS * P.r[S], PP>(dst, a, twl, tid);
+        zstage_mid<T, N, Z::NL, Z::NT, NF, P.r[S], NS, INV, !PP, LB>(a, dst, twl, tid);
+        return zstages_mid<T, N, LINES, NF, INV, S + 1, NS * P.r[S], PP, LB>(dst, a, twl, tid);
     } else {
         return a;
     }
@@ -579,8 +595,10 @@ constexpr int kLdsBytes = 160 * 1024;  // LDS per work-group on gfx950
 // first-stage inputs) is consumed and the last stage's input tile is
 // returned (its writes are complete: the barrier of the stage that wrote it
 // has passed). Template arguments as zfft.
-template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false, int A0, int B0>
-__device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>* lds, const cx<T>* twl, int tid) {
+template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false, bool LB = false,
+          int A0, int B0>
+__device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>* lds, const cx<T>* twl, int tid,
+                                                  cx<T>* lds_b = nullptr) {
     using Z = ZTile<T, N, LINES>;
     static_assert(A0 == Z::T0 && B0 == Z::R0, "slot arrays must match the tile");
     static_assert(!MIRROR || Z::kMirrorable, "mirrored first stage needs one task per thread");
@@ -594,7 +612,7 @@ __device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>*
     // previous readers of tile A are done (ping-pong with an odd number of
     // middle stages: the last reads of A were the middle stage's, before
     // the barrier that ended it)
-    if constexpr (!PP || z_mid_stages<T, N, LINES>() % 2 == 0) __syncthreads();
+    if constexpr (!PP || z_mid_stages<T, N, LINES>() % 2 == 0) zbarrier<LB>();
     // unpadded rows: lane group p = tid / NL writes element r ^ (p & SWM) in
     // write r
     constexpr bool kPow2 = (R0 & (R0 - 1)) == 0;
@@ -629,9 +647,9 @@ __device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>*
                 for (int r = 0; r < R0; ++r) row0[(r ^ sw) * S] = u[r];
             }
         }
-    __syncthreads();
+    zbarrier<LB>();
     // stages 1 .. count-2 (NS after stage 0 is R0)
-    return zstages_mid<T, N, LINES, NF, INV, 1, R0, PP>(lds, lds + NF * Z::LDS, twl, tid);
+    return zstages_mid<T, N, LINES, NF, INV, 1, R0, PP, LB>(lds, lds_b ? lds_b : lds + NF * Z::LDS, twl, tid);
 }
 
 // v: first-stage inputs (consumed), w: last-stage outputs. LINES: tile width
@@ -641,11 +659,12 @@ __device__ __forceinline__ const cx<T>* zfft_head(cx<T> (&v)[NF][A0][B0], cx<T>*
 // last stage reads tile B, so the next transform's first-stage write to tile
 // A needs no barrier either. Between PP transforms a caller may not touch lds.
 // MIRROR / MIRROR_LAST: first / last-stage butterflies in ZTile::j_mirror
-// order (w then holds the elements e_last_mirror).
+// order (w then holds the elements e_last_mirror). LB: LDS-only barriers
+// (zbarrier). lds_b: tile B of PP where it is not the one after tile A.
 template <typename T, int N, int NF, bool INV, int LINES = 0, bool PP = false, bool MIRROR = false,
-          bool MIRROR_LAST = false, int A0, int B0, int A1, int B1>
+          bool MIRROR_LAST = false, bool LB = false, int A0, int B0, int A1, int B1>
 __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][B1], cx<T>* lds, const cx<T>* twl,
-                                     int tid) {
+                                     int tid, cx<T>* lds_b = nullptr) {
     using Z = ZTile<T, N, LINES>;
     static_assert(A0 == Z::T0 && B0 == Z::R0 && A1 == Z::TLAST && B1 == Z::RL, "slot arrays must match the tile");
     static_assert(!MIRROR_LAST || Z::kMirrorableLast, "mirrored last stage needs one task per thread");
@@ -661,7 +680,7 @@ __device__ __forceinline__ void zfft(cx<T> (&v)[NF][A0][B0], cx<T> (&w)[NF][A1][
             }
     } else {
         const int l = tid % NL;
-        const cx<T>* const last = zfft_head<T, N, NF, INV, LINES, PP, MIRROR>(v, lds, twl, tid);
+        const cx<T>* const last = zfft_head<T, N, NF, INV, LINES, PP, MIRROR, LB>(v, lds, twl, tid, lds_b);
         constexpr int NSL = N / RL;
 #pragma unroll
         for (int i = 0; i < Z::TLAST; ++i) {

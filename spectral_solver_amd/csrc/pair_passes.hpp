@@ -96,6 +96,15 @@ __device__ __forceinline__ cx2<T> ld_cx2(__amdgpu_buffer_rsrc_t r, uint32_t off)
     return q;
 }
 
+// The x pass prefetches the rows of its second and third velocity pairs into
+// LDS while the previous pair is transformed (FP32, line layout, lengths with
+// padded ping-pong tiles: 256 and 512; 512^3 x pass 2.35 -> 2.27 ms).
+// SPH_X_DMA=0 builds without it.
+#ifndef SPH_X_DMA
+#define SPH_X_DMA 1
+#endif
+constexpr bool kXPrefetch = SPH_X_DMA != 0;
+
 // store of a cx2 with cache policy POL
 template <int POL, typename T>
 __device__ __forceinline__ void gst2(cx2<T>* p, cx2<T> v) {
@@ -572,9 +581,12 @@ constexpr int kVmaxStride = 16;
 // NL lines are one contiguous block of NL * (N/2+1) pairs, row k of line l at
 // l * (N/2+1) + k.
 // One tile (lds holds z_lds<T, N, LINES>() values, the twiddles are in twl).
+// lds_b: the second tile, where the kernel declares it apart from the first
+// (the prefetch into it then stays in flight across the first tile's
+// barriers); else lds + Z::LDS.
 template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileId tile, cx<T>* lds, const cx<T>* twl,
-                                                    int tid) {
+                                                    int tid, cx<T>* lds_b = nullptr) {
     using Z = ZTile<T, N, LINES>;
     constexpr int NL = Z::NL, NT = Z::NT, E = Z::E, S = Z::S;
     constexpr int T0 = Z::T0, R0 = Z::R0, TLS = Z::TLAST, RL = Z::RL;
@@ -584,7 +596,8 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     // addressing pushes the 128-register x tile further into spills (3%
     // faster at 512; at 1024 the x pass takes 33.4 ms with it, 29.1 without)
     constexpr bool PP = N <= 512 && z_pingpong<T, N, LINES>();
-    cx<T>* const sep = PP ? lds + Z::LDS : lds;
+    cx<T>* const tile_b = lds_b ? lds_b : lds + Z::LDS;
+    cx<T>* const sep = PP ? tile_b : lds;
     // inverse transforms: first-stage butterflies in mirrored order, so the
     // two reads of each spectral row (elements k and N - k) come from one
     // wavefront (ZTile::j_mirror); measured 2% faster at 512, slower at 1024
@@ -622,11 +635,49 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     // thread reads back only its own slots.
     constexpr bool PARK = !PP && z_pingpong<T, N, LINES>();
     static_assert(!PARK || E * NT <= Z::LDS, "the parking area fits the second tile");
-    cx<T>* const park = lds + Z::LDS;
+    cx<T>* const park = tile_b;
+    // Prefetch (DMA): the rows of velocity pairs 1 and 2 are loaded straight
+    // into tile B (global-to-LDS buffer loads, [k][line] cx2) while the
+    // previous pair's inverse transform runs on tile A alone with LDS-only
+    // barriers; the third transform and the forward ones use both tiles.
+    constexpr int PRE_ENT = H * NL;               // cx2 entries of one pair's rows
+    constexpr int PRE_INS = (PRE_ENT + 63) / 64;  // 64-lane load instructions
+    // (where the rows fit the second tile: the padded 16-line FP32 tiles)
+    constexpr bool DMA = kXPrefetch && sizeof(T) == 4 && PP && !XT && kNarrow && PRE_INS * 64 * 2 <= Z::LDS;
+    cx2<T>* const pre = reinterpret_cast<cx2<T>*>(tile_b);
+    auto prefetch = [&](int c) {
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + static_cast<size_t>(z) * a.ny + y0;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<cx2<T>*>(AB), 0, static_cast<int>(((H - 1) * row + NL) * sizeof(cx2<T>)), 0x00020000);
+        // a fixed, unrolled count per wave (the remainder under a uniform
+        // branch), so that the compiler's waits for the first pair's own
+        // loads, issued before these, leave the prefetch in flight
+        constexpr int NW = NT / 64;
+        constexpr int FULL = PRE_INS / NW;
+        const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
+        auto issue = [&](int q) {
+            const int e = q * 64 + lane;
+            const int k = e / NL, ll = e % NL;
+            // rows past the last line and the padding lanes read out of range: zeros
+            const uint32_t off = e < PRE_ENT && y0 + ll < a.ny
+                                     ? (static_cast<uint32_t>(k) * row + ll) * static_cast<uint32_t>(sizeof(cx2<T>))
+                                     : 0xFFFFFFC0u;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(pre + q * 64), 16,
+                                                     off, 0, 0, XPOL);
+        };
+#pragma unroll
+        for (int it = 0; it < FULL; ++it) issue(wave + it * NW);
+        if (FULL * NW + wave < PRE_INS) issue(FULL * NW + wave);
+    };
     auto uval = [&](int c, int m) -> T { return PARK && c == 0 ? park[m * NT + tid].x : u[c][m]; };
     auto sval = [&](int c, int m) -> T { return PARK && c == 0 ? park[m * NT + tid].y : s[c][m]; };
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
+        if (DMA && c > 0) {
+            // pair c's rows are in tile B in every wave
+            __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+            __syncthreads();
+        }
         const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + (XT ? xt_block : static_cast<size_t>(z) * a.ny + y0);
         // XT: one resource for the tile's block; lanes past the last line read zeros
         const __amdgpu_buffer_rsrc_t rsx = __builtin_amdgcn_make_buffer_rsrc(
@@ -647,7 +698,9 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 const int e = MIR ? Z::e_first_mirror(tid, r) : Z::e_first(tid, i, r);
                 const int k = e <= N / 2 ? e : N - e;
                 cx2<T> q;
-                if constexpr (XT) {
+                if (DMA && c > 0) {
+                    q = ok ? pre[k * NL + l] : cx2<T>{};
+                } else if constexpr (XT) {
                     const uint32_t off = ok ? (l * H + k) * static_cast<uint32_t>(sizeof(cx2<T>)) : 0xFFFFFFC0u;
                     q = ld_cx2<T, XPOL>(rsx, off);
                 } else if constexpr (kNarrow) {
@@ -668,7 +721,15 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 }
                 v[0][i][r] = e <= N / 2 ? cx<T>{va.x - vb.y, va.y + vb.x} : cx<T>{va.x + vb.y, vb.x - va.y};
             }
-        zfft<T, N, 1, true, LINES, PP, MIR>(v, w, lds, twl, tid);
+        if (DMA && c == 0) prefetch(1);
+        if (DMA && c == 1) {
+            __syncthreads();  // every wave has its pair-1 rows out of tile B
+            prefetch(2);
+        }
+        if (DMA && c < 2)
+            zfft<T, N, 1, true, LINES, false, MIR, false, true>(v, w, lds, twl, tid);
+        else
+            zfft<T, N, 1, true, LINES, PP, MIR>(v, w, lds, twl, tid, tile_b);
 #pragma unroll
         for (int i = 0; i < TLS; ++i)
 #pragma unroll
@@ -711,7 +772,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                 const int m = i + r * T0;
                 v[0][i][r] = {uval(PA[p], m) * uval(PB[p], m), sval(PA[p], m) * sval(PB[p], m)};
             }
-        zfft<T, N, 1, false, LINES, PP, false, MIRL>(v, w, lds, twl, tid);
+        zfft<T, N, 1, false, LINES, PP, false, MIRL>(v, w, lds, twl, tid, tile_b);
         if constexpr (MIRL) {
             // element j + r*STEP for r < RL/2 (and N/2 in group j = 0): rows
             // k = 0 .. N/2 once over the work-group
@@ -776,13 +837,18 @@ template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 // at 128 like the 1024-thread form.
 __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
     k_px_products_rd(PairXArgs<T> a) {
-    __shared__ cx<T> lds[z_lds<T, N, LINES>()];
+    using Z = ZTile<T, N, LINES>;
+    constexpr bool kTwoTiles = z_lds<T, N, LINES>() == 2 * Z::LDS;
+    // the two tiles as separate arrays, so that the compiler sees the
+    // prefetch into the second one apart from the first one's accesses
+    __shared__ cx<T> lds[Z::LDS];
+    __shared__ cx<T> lds_b[kTwoTiles ? Z::LDS : 1];
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
-    load_twiddles<T, N, ZTile<T, N, LINES>::NT>(twl, a.tw, tid);
+    load_twiddles<T, N, Z::NT>(twl, a.tw, tid);
     // consecutive tiles of an XCD: neighbouring y blocks of one z, i.e. the
     // same translation pages of every k row
-    px_products_rd_tile<T, N, LINES, VMAX, XT>(a, xcd_tile_id(), lds, twl, tid);
+    px_products_rd_tile<T, N, LINES, VMAX, XT>(a, xcd_tile_id(), lds, twl, tid, kTwoTiles ? lds_b : nullptr);
 }
 
 // LDS-staged form: each transform stages its whole tile through LDS.
