@@ -7,6 +7,7 @@ written next to this file so that they travel with the source tree.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -62,14 +63,27 @@ def _run(cmds):
         raise RuntimeError("build failed:\n" + msg)
 
 
+def build_id() -> str:
+    """Hash of the library's sources and public headers: the autotuner's
+    cache file (SPH_TUNE_CACHE) keys its entries by it."""
+    h = hashlib.sha1()
+    for d in (CSRC, INCLUDE):
+        for name in sorted(os.listdir(d)):
+            if name.endswith((".hip", ".hpp", ".h")):
+                with open(os.path.join(d, name), "rb") as f:
+                    h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     cmds = []
+    bid = '-DSPH_BUILD_ID="' + build_id() + '"'
     for u in UNITS:
         obj = os.path.join(BUILD, u.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, u), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, bid, "-c", os.path.join(CSRC, u), "-o", obj])
     if verbose:
         print("[build] compiling", ", ".join(UNITS), "for", ARCH, flush=True)
     _run(cmds)

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -27,6 +28,12 @@
 
 #include "common.hpp"
 #include "ensemble_passes.hpp"
+
+// identifies the library build in the autotuner's cache file (build.py
+// passes a hash of the sources)
+#ifndef SPH_BUILD_ID
+#define SPH_BUILD_ID "unversioned"
+#endif
 #include "spectral_hip.h"
 
 namespace sph {
@@ -595,13 +602,115 @@ private:
     // the fastest is kept. Results are cached per (precision, n, members) for
     // the process; SPH_AUTOTUNE=0 keeps variant 0 everywhere,
     // SPH_TILE_VARIANT=v forces variant v wherever it exists.
-    void autotune() {
-        using Key = std::tuple<int, int, int>;
-        struct Choice {
-            int var[kTuned], xvar[kTuned];
-            double ms[kTuned][kMaxVariants], xms[kTuned][kMaxVariants], layout_ms[2];
-            bool xt;
+    struct Choice {
+        int var[kTuned], xvar[kTuned];
+        double ms[kTuned][kMaxVariants], xms[kTuned][kMaxVariants], layout_ms[2];
+        bool xt;
+    };
+    void apply(const Choice& c) {
+        for (int p = 0; p < kTuned; ++p) {
+            var_[p] = c.var[p];
+            xvar_[p] = c.xvar[p];
+            for (int v = 0; v < kMaxVariants; ++v) {
+                tune_ms_[p][v] = c.ms[p][v];
+                xtune_ms_[p][v] = c.xms[p][v];
+            }
+        }
+        layout_ms_[0] = c.layout_ms[0];
+        layout_ms_[1] = c.layout_ms[1];
+        use_transposed(c.xt);
+    }
+    // The autotuner's choices across processes: with SPH_TUNE_CACHE set to a
+    // file path, a choice is looked up there by (library build, device,
+    // precision, n, members, forced layout) before timing, and a new one is
+    // appended after it. One line per choice:
+    //   sph-tune-1 <build> <device> <bytes> <n> <members> <layout> <xt>
+    //     <var x4> <xvar x4> <layout ms x2> <ms x16> <xms x16>
+    // (the last matching line wins). Entries whose variants do not exist in
+    // this build or whose layout does not fit are ignored.
+    static std::string tune_key(int forced_layout, int n_, int members) {
+        int dev = 0;
+        hipDeviceProp_t prop{};
+        std::string device = "unknown";
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            device = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+        for (char& ch : device)
+            if (ch == ' ') ch = '_';
+        return std::string("sph-tune-1 ") + SPH_BUILD_ID + " " + device + " " + std::to_string(sizeof(T)) + " " +
+               std::to_string(n_) + " " + std::to_string(members) + " " + std::to_string(forced_layout);
+    }
+    bool choice_valid(const Choice& c, bool xt_allowed) const {
+        if (c.xt && !xt_allowed) return false;
+        for (int p = 0; p < kTuned; ++p) {
+            if (c.var[p] < 0 || c.var[p] >= pt_->variants[p]) return false;
+            if (c.xvar[p] < 0 || c.xvar[p] >= kMaxVariants) return false;
+        }
+        return true;
+    }
+    static bool load_tune_file(const char* path, const std::string& key, Choice& out) {
+        std::FILE* f = std::fopen(path, "r");
+        if (!f) return false;
+        bool found = false;
+        char line[4096];
+        while (std::fgets(line, sizeof(line), f)) {
+            const std::size_t klen = key.size();
+            if (std::strncmp(line, key.c_str(), klen) != 0 || line[klen] != ' ') continue;
+            Choice c{};
+            const char* q = line + klen;
+            char* end = nullptr;
+            auto next = [&](double& v) {
+                v = std::strtod(q, &end);
+                if (end == q) return false;
+                q = end;
+                return true;
+            };
+            double v = 0.0;
+            bool ok = next(v);
+            c.xt = v != 0.0;
+            for (int p = 0; ok && p < kTuned; ++p) {
+                ok = next(v);
+                c.var[p] = static_cast<int>(v);
+            }
+            for (int p = 0; ok && p < kTuned; ++p) {
+                ok = next(v);
+                c.xvar[p] = static_cast<int>(v);
+            }
+            for (int i = 0; ok && i < 2; ++i) ok = next(c.layout_ms[i]);
+            for (int p = 0; ok && p < kTuned; ++p)
+                for (int i = 0; ok && i < kMaxVariants; ++i) ok = next(c.ms[p][i]);
+            for (int p = 0; ok && p < kTuned; ++p)
+                for (int i = 0; ok && i < kMaxVariants; ++i) ok = next(c.xms[p][i]);
+            if (ok) {
+                out = c;
+                found = true;
+            }
+        }
+        std::fclose(f);
+        return found;
+    }
+    static void append_tune_file(const char* path, const std::string& key, const Choice& c) {
+        std::string line = key + " " + (c.xt ? "1" : "0");
+        char buf[64];
+        for (int p = 0; p < kTuned; ++p) line += " " + std::to_string(c.var[p]);
+        for (int p = 0; p < kTuned; ++p) line += " " + std::to_string(c.xvar[p]);
+        auto add = [&](double v) {
+            std::snprintf(buf, sizeof(buf), " %.6g", v);
+            line += buf;
         };
+        add(c.layout_ms[0]);
+        add(c.layout_ms[1]);
+        for (int p = 0; p < kTuned; ++p)
+            for (int i = 0; i < kMaxVariants; ++i) add(c.ms[p][i]);
+        for (int p = 0; p < kTuned; ++p)
+            for (int i = 0; i < kMaxVariants; ++i) add(c.xms[p][i]);
+        line += "\n";
+        // one write in append mode: concurrent processes add whole lines
+        if (std::FILE* f = std::fopen(path, "a")) {
+            std::fputs(line.c_str(), f);
+            std::fclose(f);
+        }
+    }
+    void autotune() {
         // SPH_LAYOUT=0 / 1 forces the line / transposed work layout (the
         // latter where the length supports it)
         const char* lay = std::getenv("SPH_LAYOUT");
@@ -627,18 +736,7 @@ private:
             std::lock_guard<std::mutex> lock(mu);
             const auto it = cache.find(key);
             if (it != cache.end()) {
-                const Choice& c = it->second;
-                for (int p = 0; p < kTuned; ++p) {
-                    var_[p] = c.var[p];
-                    xvar_[p] = c.xvar[p];
-                    for (int v = 0; v < kMaxVariants; ++v) {
-                        tune_ms_[p][v] = c.ms[p][v];
-                        xtune_ms_[p][v] = c.xms[p][v];
-                    }
-                }
-                layout_ms_[0] = c.layout_ms[0];
-                layout_ms_[1] = c.layout_ms[1];
-                use_transposed(c.xt);
+                apply(it->second);
                 return;
             }
         }
@@ -646,6 +744,17 @@ private:
         if (env && env[0] == '0') {
             use_transposed(forced_layout == 1 && xt_allowed);
             return;
+        }
+        const char* tune_file = std::getenv("SPH_TUNE_CACHE");
+        const std::string file_key = tune_file && tune_file[0] ? tune_key(forced_layout, n, count) : std::string();
+        if (!file_key.empty()) {
+            Choice c{};
+            if (load_tune_file(tune_file, file_key, c) && choice_valid(c, xt_allowed)) {
+                apply(c);
+                std::lock_guard<std::mutex> lock(mu);
+                cache.emplace(key, c);
+                return;
+            }
         }
         Stream stream;
         const hipStream_t s = stream.get();
@@ -708,6 +817,7 @@ private:
         c.layout_ms[0] = layout_ms_[0];
         c.layout_ms[1] = layout_ms_[1];
         c.xt = keep_xt;
+        if (!file_key.empty()) append_tune_file(tune_file, file_key, c);
         std::lock_guard<std::mutex> lock(mu);
         cache.emplace(key, c);
     }
