@@ -3,9 +3,9 @@ nu = 1/1600, dt = 5e-4, t = 0 -> 2 (4000 RK3 steps), the FP32 form (FP32
 transforms and pointwise work, FP64 reductions) and the plain FP64 path side
 by side on the GPU. The maximum over the outputs (every 0.1) of the relative
 error of kinetic energy, dissipation and enstrophy against FP64 must be at
-most 1e-4 (the north-star bound on the dissipation; the reference's own FP32
-runs differ from FP64 by 2e-5 to 9e-5 at t = 20, Benchmark.md:619/626,
-745/752, 773/780). Measured on the 512^3 run of the same configuration:
+most 1e-4 (the error allowed to the FP32 form's dissipation rate; the
+reference's own FP32 runs differ from FP64 by 2e-5 to 9e-5 at t = 20,
+Benchmark.md:619/626, 745/752, 773/780). Measured on the 512^3 run of the same configuration:
 6.8e-7 (E), 5.8e-7 (dissipation and enstrophy).
 """
 import time
