@@ -408,6 +408,13 @@ def main():
                 "isolated_launch_ms": round(iso_ms, 4) if iso_ms else None,
                 "isolated_frac": round(by / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if iso_ms else None,
                 "algorithmic_bytes": by,
+                "algorithmic_bytes_note": ("the five-pass model's bytes per launch of this pass; for the forward z + "
+                                           "update pass 30C, of which stage 0 needs 27C (no g to read), so over "
+                                           "a step's three launches the mean is 29C: frac_stage_mean_bytes"
+                                           if PASS_NAMES[dom] == "forward_z_update" else
+                                           "the five-pass model's bytes per launch of this pass"),
+                "frac_stage_mean_bytes": (round(by * 29.0 / 30.0 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                          if PASS_NAMES[dom] == "forward_z_update" else None),
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "step_frac_implemented_bytes": round(implemented_step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "passes_in_step_ms": {k: round(instep[k][0], 4) for k in PASS_NAMES[1:]},
