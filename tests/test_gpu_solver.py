@@ -169,6 +169,32 @@ def test_published_fp64_end_state_at_t20(tmp_path, n):
     assert devs["dissipation"].max_rel < tol_eps and devs["enstrophy"].max_rel < tol_eps
 
 
+# the reference's own FP32 (cuFFT) runs of the same configurations
+# (Benchmark.md:619-622 at 64^3, :633-636 at 128^3): E and eps at t = 20
+PUBLISHED_T20_FP32 = {64: (0.016877809604920913, 0.0012350909170767454),
+                      128: (0.021553486211075779, 0.001902701557113874)}
+PUBLISHED_T20_FP64 = {64: (0.01687796507292055, 0.0012350569438618611),
+                      128: PUBLISHED_T20[128][:2]}
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_fp32_form_at_t20_against_published_finals(tmp_path, n):
+    """The FP32 form through the adaptive-CFL driver to t = 20: its E and eps
+    lie within 1e-4 of the published FP64 finals (the bound on the FP32
+    form's dissipation error; the reference's own FP32 runs are 3e-6 to 5e-5
+    away at these sizes) and within 1e-4 of the reference's FP32 finals."""
+    print("case t=20 fp32 n", n, flush=True)
+    cfg = sph.RunConfig(n=n, adaptive_cfl=True, dt=0.5, cfl=0.5, t_end=20.0, output_dt=1.0, reynolds=1600.0,
+                        output_dir=str(tmp_path), precision="fp32", print_every=250)
+    d = sph.run_taylor_green(cfg)
+    assert abs(d.t - 20.0) < 1e-12
+    for name, ref in (("FP64", PUBLISHED_T20_FP64[n]), ("reference FP32", PUBLISHED_T20_FP32[n])):
+        e_err = abs(d.kinetic_energy - ref[0]) / ref[0]
+        eps_err = abs(d.dissipation - ref[1]) / ref[1]
+        print(f"  vs published {name}: E rel {e_err:.2e}, eps rel {eps_err:.2e}", flush=True)
+        assert e_err < 1e-4 and eps_err < 1e-4
+
+
 def test_fp32_form_error_against_fp64():
     n, dt, steps = 64, 1e-3, 100
     g32 = sph.TaylorGreenSolver(n, 1600.0, "fp32")
