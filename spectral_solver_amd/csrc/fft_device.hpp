@@ -146,10 +146,13 @@ constexpr RadixPlan z_radix_plan(int n, int order = 0) {
 
 // Alternative stage order of the z-type passes, timed by the autotuner
 // against z_radix_plan(n):
-//   powers of two: the radix-8 stages last and radix-4 stages first
-//     (1024 = 4*4*8*8, 256 = 4*8*8, 128 = 4*4*8, 32 = 4*8); a plan of three
-//     or more radix-8 stages only (512 = 8*8*8) instead takes its first two
-//     as 4*4*4 (512 = 4*4*4*8): one stage more, a radix-4 first stage;
+//   1024: 8*16*8, three stages instead of four, with a radix-16 middle stage
+//     (16 values per thread: the tile runs one 512-thread work-group per CU
+//     with 256 registers);
+//   other powers of two: the radix-8 stages last and radix-4 stages first
+//     (256 = 4*8*8, 128 = 4*4*8, 32 = 4*8); a plan of three or more radix-8
+//     stages only (512 = 8*8*8) instead takes its first two as 4*4*4 (512 =
+//     4*4*4*8): one stage more, a radix-4 first stage;
 //   lengths with a factor 3 or 5: the default plan's last stage first
 //     (96 = 6*4*4, 48 = 3*4*4, 384 = 6*4*4*4, 80 = 5*4*4).
 // Where this gives the default plan again (64 = 8*8, 16 = 4*4, 8), the
@@ -157,6 +160,15 @@ constexpr RadixPlan z_radix_plan(int n, int order = 0) {
 constexpr RadixPlan z_radix_plan_alt(int n) {
     const RadixPlan p = z_radix_plan(n, 0);
     if (p.count < 2) return p;
+    if (n == 1024) {
+        // three stages with a radix-16 middle: 16 values per thread, twice
+        // the registers of the default tile (ZTile::kMinWaves)
+        RadixPlan q{};
+        q.r[q.count++] = 8;
+        q.r[q.count++] = 16;
+        q.r[q.count++] = 8;
+        return q;
+    }
     bool pow2 = true;
     int eights = 0;
     for (int s = 0; s < p.count; ++s) {
@@ -222,7 +234,7 @@ struct Tile {
 // Register DFTs. rot() multiplies by -i (forward) or +i (inverse).
 // ---------------------------------------------------------------------------
 template <bool INV, typename T>
-__device__ __forceinline__ cx<T> rot(cx<T> a) {
+__host__ __device__ __forceinline__ cx<T> rot(cx<T> a) {
     if constexpr (INV) {
         return {-a.y, a.x};
     } else {
@@ -231,7 +243,7 @@ __device__ __forceinline__ cx<T> rot(cx<T> a) {
 }
 
 template <int R, bool INV, typename T>
-__device__ __forceinline__ void dft(cx<T>* v) {
+__host__ __device__ __forceinline__ void dft(cx<T>* v) {
     if constexpr (R == 2) {
         const cx<T> a = v[0];
         v[0] = a + v[1];
@@ -289,6 +301,41 @@ __device__ __forceinline__ void dft(cx<T>* v) {
         v[4] = a3[1] - t1;
         v[2] = a3[2] + t2;
         v[5] = a3[2] - t2;
+    } else if constexpr (R == 16) {
+        // 16 = 4 x 4: DFT4 over x[m + 4 q] for each m, twiddles w16^(m k1),
+        // DFT4 over m: y[k1 + 4 k2] = sum_m w4^(m k2) w16^(m k1) DFT4_m[k1]
+        const T c1 = T(0.923879532511286756128183189396788933L);  // cos(pi/8)
+        const T s1 = T(0.382683432365089771728459984030398866L);  // sin(pi/8)
+        const T h = T(0.707106781186547524400844362104849039L);
+        cx<T> a[4][4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            cx<T> q[4] = {v[m], v[m + 4], v[m + 8], v[m + 12]};
+            dft<4, INV>(q);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a[m][k] = q[k];
+        }
+        // w16^j = (cos(pi j/8), -+sin(pi j/8)) for j = m k1 in {1, 2, 3, 4, 6, 9}
+        auto tw = [&](cx<T> x, T c, T sn) -> cx<T> {
+            const cx<T> w = INV ? cx<T>{c, sn} : cx<T>{c, -sn};
+            return cmul(x, w);
+        };
+        a[1][1] = tw(a[1][1], c1, s1);
+        a[1][2] = tw(a[1][2], h, h);
+        a[1][3] = tw(a[1][3], s1, c1);
+        a[2][1] = tw(a[2][1], h, h);
+        a[2][2] = rot<INV>(a[2][2]);
+        a[2][3] = tw(a[2][3], -h, h);
+        a[3][1] = tw(a[3][1], s1, c1);
+        a[3][2] = tw(a[3][2], -h, h);
+        a[3][3] = tw(a[3][3], -c1, -s1);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cx<T> q[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            dft<4, INV>(q);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = q[k2];
+        }
     } else if constexpr (R == 5) {
         // y_k = sum_q x_q w^{qk}, w = e^{-+2 pi i/5}
         const T c1 = T(0.309016994374947424102293417182819059L);   // cos(2pi/5)
@@ -471,6 +518,11 @@ struct ZTile {
     static constexpr int NL = NL_ ? NL_ : pick_nl();
     static constexpr int E = lcm_radix();  // elements per thread and field
     static constexpr int NT = NL * N / E;
+    // minimum waves per SIMD of the kernels on this tile (their launch
+    // bounds): 512-thread tiles ask for 4 (two work-groups per CU, 128
+    // registers), tiles of 16 or more values per thread for 2 (one
+    // work-group per CU, 256 registers), others for 1
+    static constexpr int kMinWaves = E >= 16 ? 2 : (NT == 512 ? 4 : 1);
     static constexpr int S = ztile_stride<T>(NL);  // LDS row stride
     static constexpr int LDS = S * N;          // complex values per field tile
     static constexpr int R0 = P.r[0];

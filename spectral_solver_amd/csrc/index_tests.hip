@@ -2,6 +2,7 @@
 // and every permutation of butterflies or slots must be a bijection, or a
 // kernel would silently overwrite or skip values. Built by build.py, run by
 // tests/test_index_maps.py (no GPU needed).
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -121,6 +122,37 @@ void xor_orders() {
         }
 }
 
+// the register DFTs against the defining sum (long double twiddles) on
+// seeded inputs, both directions
+template <int R>
+void register_dft() {
+    unsigned long long state = 12345 + R;
+    auto rnd = [&]() {
+        state = state * 6364136223846793005ULL + 1442695040888963407ULL;
+        return static_cast<double>(state >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    };
+    for (int inv = 0; inv < 2; ++inv) {
+        sph::cx<double> v[R], x[R];
+        for (int i = 0; i < R; ++i) x[i] = v[i] = {rnd(), rnd()};
+        if (inv)
+            sph::dft<R, true>(v);
+        else
+            sph::dft<R, false>(v);
+        double worst = 0.0;
+        for (int k = 0; k < R; ++k) {
+            long double re = 0, im = 0;
+            for (int j = 0; j < R; ++j) {
+                const long double a = (inv ? 2.0L : -2.0L) * 3.14159265358979323846264338327950288L * j * k / R;
+                re += x[j].x * cosl(a) - x[j].y * sinl(a);
+                im += x[j].x * sinl(a) + x[j].y * cosl(a);
+            }
+            const double e = std::fabs(static_cast<double>(re) - v[k].x) + std::fabs(static_cast<double>(im) - v[k].y);
+            worst = e > worst ? e : worst;
+        }
+        expect(worst < 1e-14, "register dft against the defining sum", R, inv);
+    }
+}
+
 }  // namespace
 
 int main() {
@@ -157,6 +189,13 @@ int main() {
     mirror_last<double, 256, 0>();
     mirror_last<double, 512, 0>();
     xor_orders();
+    register_dft<2>();
+    register_dft<3>();
+    register_dft<4>();
+    register_dft<5>();
+    register_dft<6>();
+    register_dft<8>();
+    register_dft<16>();
     if (failures) {
         std::printf("%d index-map checks failed\n", failures);
         return 1;

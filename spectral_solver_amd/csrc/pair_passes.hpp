@@ -283,7 +283,7 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
 // tiles; smaller tiles leave the register budget to the compiler: held to
 // 128 the 96^3 forward-z kernel spills 240 bytes per lane)
 template <typename T, int N, int MODE, int LINES = 0>
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::kMinWaves))
     k_pinv_z(PairInvZArgs<T> a) {
     using Z = ZTile<T, N, LINES>;
     __shared__ cx<T> lds[z_lds<T, N, LINES>()];
@@ -839,7 +839,7 @@ template <typename T, int N, int LINES = 0, bool VMAX = false, bool XT = false>
 // The second bound is the minimum number of waves per SIMD: 512-thread tiles
 // ask for 4, i.e. two work-groups per CU (16 waves), which caps the registers
 // at 128 like the 1024-thread form.
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::kMinWaves))
     k_px_products_rd(PairXArgs<T> a) {
     using Z = ZTile<T, N, LINES>;
     constexpr bool kTwoTiles = z_lds<T, N, LINES>() == 2 * Z::LDS;
@@ -1052,7 +1052,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
 // inv.out[0..2] (the first three product pair fields, whose values the
 // forward half has consumed: every work-group reads its own lines only).
 template <typename T, int N, int LINES = 0>
-__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::NT == 512 ? 4 : 1))
+__global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>::kMinWaves))
     k_pfwd_z_rk_inv(PairFwdZArgs<T> a, PairInvZArgs<T> inv) {
     using Z = ZTile<T, N, LINES>;
     __shared__ cx<T> lds[z_lds<T, N, LINES>()];

@@ -43,22 +43,22 @@ def test_supported_lengths():
 
 def test_radix_plans_of_the_z_type_passes():
     """Both radix orders of every compiled length factor the length into the
-    butterflies the kernels have (2, 3, 4, 5, 6, 8); the alternative order is
-    the stated permutation of the default's stages (or, for all-radix-8
-    plans, the first two as 4*4*4), and differs from it wherever the
-    autotuner offers it."""
+    butterflies the kernels have (2, 3, 4, 5, 6, 8, 16); the alternative
+    order is the stated permutation of the default's stages (or, for
+    all-radix-8 plans, the first two as 4*4*4; for 1024 the three-stage plan
+    8*16*8), and differs from it wherever the autotuner offers it."""
     lengths = (8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024)
-    expect = {1024: ([8, 4, 4, 8], [4, 4, 8, 8]), 512: ([8, 8, 8], [4, 4, 4, 8]), 256: ([8, 4, 8], [4, 8, 8]),
+    expect = {1024: ([8, 4, 4, 8], [8, 16, 8]), 512: ([8, 8, 8], [4, 4, 4, 8]), 256: ([8, 4, 8], [4, 8, 8]),
               128: ([8, 4, 4], [4, 4, 8]), 96: ([4, 4, 6], [6, 4, 4]), 80: ([4, 4, 5], [5, 4, 4]),
               64: ([8, 8], [8, 8]), 48: ([4, 4, 3], [3, 4, 4]), 32: ([8, 4], [4, 8])}
     for n in lengths:
         a, b = sph.radix_plan(n, 0), sph.radix_plan(n, 1)
         for plan in (a, b):
-            assert set(plan) <= {2, 3, 4, 5, 6, 8}
+            assert set(plan) <= {2, 3, 4, 5, 6, 8, 16}
             assert int(np.prod(plan)) == n, (n, plan)
         if n in expect:
             assert (a, b) == expect[n]
-        if not all(r == 8 for r in a):
+        if not all(r == 8 for r in a) and n != 1024:
             assert sorted(a) == sorted(b), (n, a, b)
     assert sph.radix_plan(100, 0) == [] and sph.radix_plan(2048, 1) == []
     with pytest.raises(RuntimeError, match="radix order"):

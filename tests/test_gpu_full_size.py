@@ -91,3 +91,30 @@ def test_full_size_step_matches_coarse_oracle(n, precision, dt, tol, components)
         assert err <= tol
     finally:
         s.close()
+
+
+def test_full_size_1024_alternative_radix_order(monkeypatch):
+    """1024^3 FP32 with the z-type passes in the alternative radix order
+    (8*16*8: a radix-16 middle stage, 16 values per thread), forced through
+    SPH_TILE_VARIANT=3, against the same 64^3 oracle as above."""
+    n, dt, tol = 1024, 2.5e-4, 2e-6
+    monkeypatch.setenv("SPH_TILE_VARIANT", "3")
+    print("case full size n=1024 fp32, alternative radix order", flush=True)
+    d_ref, modes_ref = coarse_reference(dt)
+    s = sph.TaylorGreenSolver(n, RE, "fp32")
+    try:
+        plans = s.radix_plans()
+        assert plans["forward_z_update"] == [8, 16, 8] and plans["x_products"] == [8, 16, 8]
+        for _ in range(STEPS):
+            s.step_rk3(dt)
+        d = s.diagnostics(STEPS * dt)
+        for k in ("kinetic_energy", "dissipation", "enstrophy"):
+            err = abs(getattr(d, k) - d_ref[k]) / abs(d_ref[k])
+            print(f"  {k}: rel {err:.2e}", flush=True)
+            assert err <= tol, k
+        got = low_modes(s.u_hat(0), n)
+        err = np.linalg.norm(got - modes_ref[0]) / np.linalg.norm(modes_ref[0])
+        print(f"  low modes: rel L2 {err:.2e}", flush=True)
+        assert err <= tol
+    finally:
+        s.close()
