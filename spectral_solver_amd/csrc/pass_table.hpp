@@ -131,6 +131,12 @@ struct Launch {
     static constexpr bool kAlt = z_plan_has_alt(N);
     static constexpr int ZA = zlines(0, 1);
     static constexpr int ZA_LINES = ZTile<T, N, ZA>::NL;
+    // the x pass's alternative-order variant: at 1024 in FP32 (8*16*8) at
+    // half the default width, 4 lines of 256 threads (1024^3: 19.1-19.6
+    // against 23.2 ms at the default width; the forward-z pass keeps the
+    // default width: 32.3 against 38.1 ms)
+    static constexpr int XA = N == 1024 && sizeof(T) == 4 ? zlines(ZT::NL / 2, 1) : ZA;
+    static constexpr int XA_LINES = ZTile<T, N, XA>::NL;
     template <int L>
     static void pinv_z_l(const PairInvZArgs<T>& a, int mode, int nky, int nkx, int members, hipStream_t s) {
         using ZV = ZTile<T, N, L>;
@@ -227,7 +233,7 @@ struct Launch {
                 if (variant == 1)
                     px_products_rd<X1, true>(a, ny, nz, members, s);
                 else if (kAlt && variant == kXOrderVariant)
-                    px_products_rd<ZA, true>(a, ny, nz, members, s);
+                    px_products_rd<XA, true>(a, ny, nz, members, s);
                 else
                     px_products_rd<X0, true>(a, ny, nz, members, s);
                 return;
@@ -241,7 +247,7 @@ struct Launch {
         } else if (variant == 1) {
             px_products_rd<X1>(a, ny, nz, members, s);
         } else if (kAlt && variant == kXOrderVariant) {
-            px_products_rd<ZA>(a, ny, nz, members, s);
+            px_products_rd<XA>(a, ny, nz, members, s);
         } else {
             px_products_rd<X0>(a, ny, nz, members, s);
         }
@@ -266,7 +272,7 @@ struct Launch {
                                         TL::NL,
                                         TL::NT,
                                         {3, kAlt ? 4 : 3, 3, kAlt ? 3 : (Z1 != Z0 ? 2 : 1)},
-                                        {{L0, L1, L2, L2}, {X0, X1, L0, ZA_LINES}, {L0, L1, L2, L2},
+                                        {{L0, L1, L2, L2}, {X0, X1, L0, XA_LINES}, {L0, L1, L2, L2},
                                          {Z0, Z1, ZA_LINES, ZA_LINES}},
                                         {{0, 0, 0, 0}, {0, 0, 0, kAlt ? 1 : 0}, {0, 0, 0, 0},
                                          {0, 0, kAlt ? 1 : 0, kAlt ? 1 : 0}},

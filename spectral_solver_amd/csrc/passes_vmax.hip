@@ -16,7 +16,7 @@ void px_products_vmax_l(const PairXArgs<T>& a, hipStream_t s) {
 template <typename T, int N>
 void px_products_vmax(const PairXArgs<T>& a, int variant, bool xt, hipStream_t s) {
     using LN = Launch<T, N>;
-    constexpr int L = LN::X0, L1 = LN::X1, LA = LN::ZA;
+    constexpr int L = LN::X0, L1 = LN::X1, LA = LN::XA;
     if constexpr (LN::kXT) {
         if (xt) {
             if (variant == 1)
