@@ -33,6 +33,8 @@ PUBLISHED_MPTS_512_FP32 = 512 ** 3 / 0.5694 / 1e6  # 569.4 ms per RK3 step
 PASS_NAMES = ("inverse_z", "inverse_y", "x_products", "forward_y", "forward_z_update")
 KERNEL_NAMES = ("k_pinv_z", "k_plines_inv", "k_px_products_rd", "k_plines_fwd", "k_pfwd_z_rk_inv")
 KERNEL_NAMES_TRANSPOSED = ("k_pinv_z", "k_plines_t_inv", "k_px_products_rd_xt", "k_plines_t_fwd", "k_pfwd_z_rk_inv")
+KERNEL_NAMES_BLOCKED = ("k_pinv_z", "k_plines_b_inv", "k_px_products_rd", "k_plines_b_fwd", "k_pfwd_z_rk_inv")
+LAYOUT_NAMES = ("line", "transposed", "ky-blocked")
 ENSEMBLE_SIZES = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
 
 
@@ -375,7 +377,7 @@ def main():
     tuning = s32.tuning()
     log("autotuner (variant, lines per tile, ms per variant):", tuning)
     layout = s32.layout()
-    log("work layout (transposed kept, ms of passes 1-3 line / transposed):", layout)
+    log("work layout (kept, ms of passes 1-3 line / transposed, 1-4 ky-blocked / best other):", layout)
 
     fp64 = None
     eps_err = None
@@ -394,8 +396,8 @@ def main():
 
     in_step = [k for k in range(1, len(PASS_NAMES))]  # the standalone inverse z runs only after set-up
     dom = max(in_step, key=lambda k: instep[PASS_NAMES[k]][0])
-    kernel = (KERNEL_NAMES_TRANSPOSED if layout[0] else KERNEL_NAMES)[dom]
-    if PASS_NAMES[dom] == "x_products" and tuning["x_products"][0] == 2 and not layout[0]:
+    kernel = (KERNEL_NAMES, KERNEL_NAMES_TRANSPOSED, KERNEL_NAMES_BLOCKED)[layout[0]][dom]
+    if PASS_NAMES[dom] == "x_products" and tuning["x_products"][0] == 2 and layout[0] != 1:
         kernel = "k_px_products"  # the LDS-staged form
     ms, by = instep[PASS_NAMES[dom]]
     iso_ms = passes[PASS_NAMES[dom]][0] if passes else None
@@ -422,8 +424,10 @@ def main():
                 "passes_isolated_gbs": ({k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in passes.items()}
                                         if passes else None),
                 "autotuned": {k: {"variant": v[0], "lines": v[1]} for k, v in tuning.items()},
-                "work_layout": {"transposed": layout[0], "passes_1_3_ms_line": round(layout[1][0], 4),
-                                "passes_1_3_ms_transposed": round(layout[1][1], 4)}}
+                "work_layout": {"layout": LAYOUT_NAMES[layout[0]], "passes_1_3_ms_line": round(layout[1][0], 4),
+                                "passes_1_3_ms_transposed": round(layout[1][1], 4),
+                                "passes_1_4_ms_ky_blocked": round(layout[1][2], 4),
+                                "passes_1_4_ms_best_other": round(layout[1][3], 4)}}
     s32.close()
 
     copy_gbs = sph.copy_bandwidth(2 << 30, 10)

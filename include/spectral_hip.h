@@ -232,16 +232,24 @@ int sph_solver_radix_plan(sph_solver_t solver, int pass, int* radices, int* coun
  * needed. */
 int sph_radix_plan(int n, int order, int* radices, int* count);
 
-/* The work layout the autotuner kept. Between the inverse-y pass, the x pass
- * and the forward-y pass the pair fields are either in the line layout
- * [kx][z][y] (an x tile reads n/2+1 rows n*n pairs apart) or, at lengths
- * >= 256 where a second work buffer of 12 half-spectrum fields fits, in the
- * transposed layout [z][y][kx] (an x tile is one contiguous block; the y
- * passes transpose on the way). *transposed = 1 for the latter;
- * ms[0], ms[1] = summed launch time of those three passes with each layout's
- * fastest tile variants (0 where a layout was not timed). SPH_LAYOUT=0 / 1
- * in the environment forces a layout. */
-int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms);
+/* The work layout the autotuner kept (*layout):
+ *   0  line layout: the pair fields between the y passes and the x pass are
+ *      [kx][z][y] (an x tile reads n/2+1 rows n*n pairs apart), the state
+ *      and the spectral pair fields [kx][kz][ky];
+ *   1  transposed: [z][y][kx] between the y passes and the x pass (an x
+ *      tile is one contiguous block; the y passes transpose on the way);
+ *   2  ky-blocked: the state and the spectral pair fields in blocks of kb ky
+ *      ([kx][ky/kb][kz][ky%kb], kb = the z tile's ky count, so a z tile is
+ *      one contiguous block), the x pass in the line layout (the y passes
+ *      move between the two).
+ * 1 and 2 exist at lengths >= 256 where a second work buffer of 12
+ * half-spectrum fields fits. ms (4 doubles): summed launch time of passes
+ * 1-3 (inverse y, x, forward y) with the fastest tile variants of layout 0
+ * and of layout 1, of passes 1-4 in layout 2, and of passes 1-4 with the
+ * faster of layouts 0 and 1 (0 where a layout was not timed). SPH_LAYOUT=0 /
+ * 1 / 2 in the environment forces a layout. The layout is internal: every
+ * state access of this API uses the plain mode order. */
+int sph_solver_layout(sph_solver_t solver, int* layout, double* ms);
 
 /* Device copy bandwidth: a streaming 16-byte-per-lane copy of `bytes` bytes,
  * timed with HIP events over `reps` launches; *gbs = 2*bytes/time (read +

@@ -402,11 +402,12 @@ class TaylorGreenSolver:
         return v.value, d.value, bool(f.value)
 
     def layout(self):
-        """(transposed work layout kept, [ms of passes 1-3 in the line layout,
-        in the transposed layout])."""
-        t, ms = ctypes.c_int(), np.zeros(2)
+        """(work layout kept: 0 line, 1 transposed, 2 ky-blocked, [ms of
+        passes 1-3 in the line layout, in the transposed layout, ms of passes
+        1-4 in the ky-blocked layout, with the faster of the other two])."""
+        t, ms = ctypes.c_int(), np.zeros(4)
         _check(load_library().sph_solver_layout(self._h, ctypes.byref(t), _dptr(ms)))
-        return bool(t.value), [float(x) for x in ms]
+        return int(t.value), [float(x) for x in ms]
 
     def tuning(self) -> dict:
         """Autotuner choice per tuned pass: (variant, lines per tile, ms of

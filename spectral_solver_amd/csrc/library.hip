@@ -479,12 +479,12 @@ int sph_radix_plan(int n, int order, int* radices, int* count) {
     });
 }
 
-int sph_solver_layout(sph_solver_t solver, int* transposed, double* ms) {
+int sph_solver_layout(sph_solver_t solver, int* layout, double* ms) {
     return guarded([&] {
         require_handle(solver);
-        require_pointer(transposed);
+        require_pointer(layout);
         require_pointer(ms);
-        solver->impl->layout(transposed, ms);
+        solver->impl->layout(layout, ms);
     });
 }
 

@@ -9,6 +9,11 @@
 //   pair field  [kx][kz or z][ky][2]        offset ((kx*n + kz)*ny + ky)*2 + f
 // and the state is component-interleaved:
 //   u_hat, g    [kx][kz][ky][3]             offset ((kx*n + kz)*ny + ky)*3 + c.
+// The spectral side (the state, and the pair fields between the z passes and
+// the y passes) may instead be ky-blocked, kb ky per block (the z tile's
+// NL/2), so that a z tile is one contiguous block of each field:
+//   [kx][ky/kb][kz][ky%kb][2 or 3]  mode offset kx*n*ny + (ky/kb)*n*kb + kz*kb + ky%kb
+// (kb = ny is the plain layout above). The z-pass arguments carry kb.
 // In the z passes a tile is NL/2 adjacent ky times both pair members, i.e. one
 // 128-byte row segment per element; lane l transforms member f = l & 1 of line
 // ky = l >> 1, and the two lanes of a line split the modes after the
@@ -37,6 +42,7 @@ struct PairInvZArgs {
     const cx<T>* tw;
     long long ms_u;
     long long ms_w;
+    int kb;  // ky block of the state and of the pair fields (ny: plain layout)
 };
 
 template <typename T>
@@ -70,6 +76,16 @@ struct PairFwdZArgs {
     const cx<T>* tw;
     long long ms_u;
     long long ms_w;
+    int kb;  // ky block of the state and of the pair fields (ny: plain layout)
+};
+
+// Mode offset within a kx plane of element (kz) 0 of line ky, and the
+// offset step between consecutive kz, for ky blocks of kb (kb = ny: the plain
+// layout, kz * ny + ky).
+struct ZRow {
+    uint32_t base, step;
+    __host__ __device__ __forceinline__ ZRow(uint32_t ky, uint32_t kb, uint32_t n)
+        : base((ky / kb) * n * kb + ky % kb), step(kb) {}
 };
 
 // both members of a pair at one mode: one 16-byte (FP32) / 32-byte access
@@ -180,8 +196,9 @@ __device__ __forceinline__ void pinv_z_tile(const PairInvZArgs<T>& a, TileId til
     const long long zw = tile.z * a.ms_w;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
     const Buf<T> U(a.u3 + tile.z * a.ms_u + plane * 3, N * ny * 3 * CB);
-    const uint32_t u_lane = ((tid / NL) * ny + kyc) * 3 * CB, u_row = ny * 3 * CB;
-    const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
+    const ZRow zr(kyc, a.kb > 0 ? a.kb : ny, N);
+    const uint32_t u_lane = (zr.base + (tid / NL) * zr.step) * 3 * CB, u_row = zr.step * 3 * CB;
+    const uint32_t p_lane = ((zr.base + (tid / NL) * zr.step) * 2 + f) * CB, p_row = zr.step * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
     cx<T> v[1][T0][R0], w[1][TLS][RL];
     auto uoff = [&](int i, int r, int c) -> uint32_t {
@@ -415,6 +432,86 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
                 const int g = m / (2 * N), rem = m % (2 * N);
                 if (2 * g < lines) gst<kPtLinePolicy>(dst + lin_off(g, rem), buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y lines between the line layout of the x pass and the ky-blocked spectral
+// layout of the z passes (out of place):
+//   line layout   [kx][z][y][2]                offset ((kx*n + z)*n + y)*2 + f
+//   ky-blocked    [kx][ky/KB][kz][ky%KB][2]    offset (kx*n*n + (ky/KB)*n*KB + kz*KB + ky%KB)*2 + f
+// with KB the default z tile's NL/2, so that a z tile is one contiguous block
+// of each field (at n = 1024 the z passes' 64-byte rows 16 KB apart cost them
+// their streaming rate). A tile is NL/2 pair lines of consecutive z (kz) of
+// one kx: on the line side NL*N contiguous values; on the blocked side, per ky
+// block, the tile's NL/2 rows of KB modes, NL*KB contiguous values. Thread q
+// of a side takes that side's q-th value in address order. INV: blocked
+// (spectral ky) in, line layout (physical y) out; forward: the other way.
+// Grid (pair lines / (NL/2), 1, field + nfields*member).
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+constexpr int zblock_ky() {
+    return ZTile<T, N>::NL / 2;
+}
+template <typename T, int N, bool INV, int LINES>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
+    k_plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+               const cx<T>* tw) {
+    using TL = Tile<T, N, LINES>;
+    constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, KB = zblock_ky<T, N>();
+    constexpr int G = NL / 2;       // pair lines per tile
+    constexpr int RUN = NL * KB;    // contiguous values per ky block on the blocked side
+    static_assert(NL % 2 == 0, "pair tiles need an even line count");
+    static_assert(N % G == 0 && N % KB == 0, "tiles must not straddle a kx plane or a ky block");
+    constexpr bool SWZ = plines_swizzled<T, N>(NL);
+    __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
+    const long long g0 = static_cast<long long>(tile.x) * G;  // pair line kx*N + z of the tile's first line
+    const int kx = static_cast<int>(g0 / N), z0 = static_cast<int>(g0 % N);
+    const int field = tile.z % nfields, member = tile.z / nfields;
+    const long long fo = field * field_stride + member * member_stride;
+    const cx<T>* src = in + fo;
+    cx<T>* dst = out + fo;
+    const long long line0 = g0 * 2 * N;                              // line side: the tile's first value
+    const long long blk0 = (static_cast<long long>(kx) * N * N + static_cast<long long>(z0) * KB) * 2;
+    // blocked side: q-th value of the tile in address order -> offset, ky, line
+    auto blk = [&](int q, int& ky, int& line) -> long long {
+        const int b = q / RUN, r = q % RUN;
+        const int g = r / (2 * KB), kyl = (r >> 1) % KB, f = r & 1;
+        ky = b * KB + kyl;
+        line = 2 * g + f;
+        return blk0 + static_cast<long long>(b) * N * KB * 2 + r;
+    };
+    load_twiddles<T, N, NT>(twl, tw, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int q = tid + i * NT;
+        if (TL::ELEMS % NT != 0 && q >= TL::ELEMS) break;
+        if constexpr (INV) {
+            int ky, line;
+            const long long o = blk(q, ky, line);
+            buf[tile_at<NL, SWZ>(ky, line)] = gld<kPolicyStream>(src + o);
+        } else {
+            const int g = q / (2 * N), rem = q % (2 * N);
+            buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = gld<kPolicyStream>(src + line0 + q);
+        }
+    }
+    __syncthreads();
+    fft_tile<T, N, NL, NT, INV, 0, 1, SWZ>(buf, twl, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int q = tid + i * NT;
+        if (TL::ELEMS % NT != 0 && q >= TL::ELEMS) break;
+        if constexpr (INV) {
+            const int g = q / (2 * N), rem = q % (2 * N);
+            gst<kPolicyStream>(dst + line0 + q, buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
+        } else {
+            int ky, line;
+            const long long o = blk(q, ky, line);
+            gst<kPolicyStream>(dst + o, buf[tile_at<NL, SWZ>(ky, line)]);
         }
     }
 }
@@ -914,7 +1011,8 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
     cx<T>* const g3 = a.g3 + tile.z * a.ms_u;
     const T* const coef = a.coef + tile.z * kCoefStride;
     const size_t plane = static_cast<size_t>(kx) * N * ny;
-    const uint32_t p_lane = (((tid / NL) * ny + kyc) * 2 + f) * CB, p_row = ny * 2 * CB;
+    const ZRow zr(kyc, a.kb > 0 ? a.kb : ny, N);
+    const uint32_t p_lane = ((zr.base + (tid / NL) * zr.step) * 2 + f) * CB, p_row = zr.step * 2 * CB;
     const uint32_t p_bytes = N * ny * 2 * CB;
     const int my = signed_mode(kyc, a.n);
     const T fx = T(kx), fy = T(my);
@@ -1003,7 +1101,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
     cx<T> u[HS][3], g[HS][3];
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
-        off[h] = (owned_e(h) * ny + kyc) * 3 * CB;
+        off[h] = (zr.base + owned_e(h) * zr.step) * 3 * CB;
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[h][c] = U.ld(off[h] + c * CB);
     }
@@ -1075,14 +1173,16 @@ __global__ void __launch_bounds__((ZTile<T, N, LINES>::NT), (ZTile<T, N, LINES>:
 
 // max |k . u_hat| over the component-interleaved state (impl.hpp:1167-1170)
 template <typename T>
-__global__ void __launch_bounds__(kReduceThreads) k_reduce_div3(const cx<T>* u3, int n, double* partial) {
+// (state in ky blocks of kb; kb = n: the plain layout)
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_div3(const cx<T>* u3, int n, int kb, double* partial) {
     const long long count = static_cast<long long>(n / 2 + 1) * n * n;
     double m = 0.0;
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const int ky = static_cast<int>(i % n);
-        const int kz = static_cast<int>((i / n) % n);
         const int kx = static_cast<int>(i / (static_cast<long long>(n) * n));
+        const int rem = static_cast<int>(i % (static_cast<long long>(n) * n));
+        const int kz = (rem % (n * kb)) / kb;
+        const int ky = (rem / (n * kb)) * kb + rem % kb;
         const T fx = T(kx), fy = T(signed_mode(ky, n)), fz = T(signed_mode(kz, n));
         const cx<T> d = (u3[3 * i] * fx + u3[3 * i + 1] * fy) + u3[3 * i + 2] * fz;
         m = fmax(m, static_cast<double>(sqrt(d.x * d.x + d.y * d.y)));

@@ -68,6 +68,14 @@ struct PassTable {
     int xt_lines[kMaxVariants];
     void (*plines_t)(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
                      int members, bool inverse, const cx<T>* tw, int variant, hipStream_t);
+    // ky-blocked spectral layout (k_plines_b): supported (zb), its ky block
+    // (the default z tile's NL/2), and the y pass between it and the line
+    // layout of the x pass (blocked in and line layout out when inverse;
+    // variants 0..2: the line layout's y-tile widths)
+    bool zb;
+    int zb_kb;
+    void (*plines_b)(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                     int members, bool inverse, const cx<T>* tw, int variant, hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -226,6 +234,31 @@ struct Launch {
                 plines_t_l<T0L>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
         }
     }
+    // the ky-blocked spectral layout, at the lengths where the z passes'
+    // rows lie far apart
+    static constexpr bool kZB = N >= 256;
+    template <int L>
+    static void plines_b_l(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                           int members, bool inverse, const cx<T>* tw, hipStream_t s) {
+        using TV = Tile<T, N, L>;
+        constexpr long long H = N / 2 + 1;
+        const dim3 grid(static_cast<unsigned>(H * N / (L / 2)), 1, nfields * members);
+        if (inverse)
+            k_plines_b<T, N, true, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+        else
+            k_plines_b<T, N, false, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+    }
+    static void plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+                         int members, bool inverse, const cx<T>* tw, int variant, hipStream_t s) {
+        if constexpr (kZB) {
+            if (variant == 2)
+                plines_b_l<L2>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+            else if (variant == 1)
+                plines_b_l<L1>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+            else
+                plines_b_l<L0>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+        }
+    }
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, bool xt,
                             hipStream_t s) {
         if constexpr (kXT) {
@@ -289,7 +322,10 @@ struct Launch {
                                         &pfwd_z_rk,
                                         kXT,
                                         {T0L, T1L, T2L, T2L},
-                                        &plines_t};
+                                        &plines_t,
+                                        kZB,
+                                        zblock_ky<T, N>(),
+                                        &plines_b};
 };
 
 }  // namespace sph

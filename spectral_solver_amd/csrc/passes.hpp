@@ -169,6 +169,7 @@ struct FwdZArgs {
     int ny;
     int n;          // cube size for the modes (PROJECT)
     const cx<T>* tw;
+    int kb;         // PROJECT into u3: ky block of the state (0 or ny: plain layout)
 };
 
 // ---------------------------------------------------------------------------
@@ -446,7 +447,9 @@ __global__ void __launch_bounds__((ZTile<T, N>::NT)) k_fwd_z(FwdZArgs<T> a) {
                     project_or_zero(u[0][i][r], u[1][i][r], u[2][i][r], kx, my, mz, k2, keep_mode(k2, a.n));
                     if (a.u3) {
                         const Buf<T> o(a.u3 + plane * 3, plane_bytes * 3);
-                        const uint32_t off3 = ((Z::e_last(tid, i, r)) * a.ny + ky) * 3 * sizeof(cx<T>);
+                        const uint32_t kb = a.kb > 0 ? a.kb : a.ny;
+                        const uint32_t off3 =
+                            ((ky / kb) * N * kb + Z::e_last(tid, i, r) * kb + ky % kb) * 3 * sizeof(cx<T>);
 #pragma unroll
                         for (int c = 0; c < 3; ++c) o.st(off3 + c * sizeof(cx<T>), u[c][i][r]);
                     } else {

@@ -189,6 +189,7 @@ public:
         nu_.resize(count);
         for (int m = 0; m < count; ++m) nu_[m] = static_cast<T>(nu[m]);
         last_dt_.assign(count, std::nan(""));
+        kb_ = n;
         autotune();
     }
 
@@ -207,28 +208,35 @@ public:
     void tuned_pass(int p, int* variant, int* lines, double* ms) const {
         if (p < 0 || p >= kTuned) throw std::runtime_error("tuned pass out of range");
         const bool xt = xt_ && p < 3;  // the forward-z pass is the same in both layouts
-        const int v = xt ? xvar_[p] : var_[p];
+        const int v = cur_var(p);
         *variant = v;
         *lines = xt && p != 1 ? pt_->xt_lines[v] : pt_->tile_lines[p][v];
-        for (int q = 0; q < kMaxVariants; ++q) ms[q] = xt ? xtune_ms_[p][q] : tune_ms_[p][q];
+        for (int q = 0; q < kMaxVariants; ++q) ms[q] = zb_ ? btune_ms_[p][q] : (xt ? xtune_ms_[p][q] : tune_ms_[p][q]);
     }
+    // the variant of tuned pass p in the layout in use
+    int cur_var(int p) const { return zb_ ? bvar_[p] : (xt_ && p < 3 ? xvar_[p] : var_[p]); }
     // the radix stages of the kept variant of tuned pass p (the y passes and
     // the LDS-staged x pass use radix_plan, the z-type passes their order)
     RadixPlan radix_plan_of(int p) const {
         if (p < 0 || p >= kTuned) throw std::runtime_error("tuned pass out of range");
         const bool xt = xt_ && p < 3;
-        const int v = xt ? xvar_[p] : var_[p];
+        const int v = cur_var(p);
         if (p == 0 || p == 2 || (p == 1 && !xt && v == kXLdsVariant)) return radix_plan(n);
         return z_radix_plan(n, pt_->radix_order[p][v]);
     }
-    // the work layout kept (1: transposed between the y passes and the x
-    // pass) and the summed time of passes 1-3 with each layout's fastest
-    // variants (0 where a layout was not timed)
-    void layout(int* transposed, double* ms) const {
-        *transposed = xt_ ? 1 : 0;
-        ms[0] = layout_ms_[0];
-        ms[1] = layout_ms_[1];
+    // the work layout kept (0: line layout, 1: transposed between the y
+    // passes and the x pass, 2: ky-blocked spectral side) and the timings
+    // it was chosen by (0 where a layout was not timed): ms[0], ms[1] the
+    // summed time of passes 1-3 with the line / transposed layout's fastest
+    // variants, ms[2] passes 1-4 in the ky-blocked layout, ms[3] passes 1-4
+    // with the faster of the other two (the forward-z pass is the same in
+    // those)
+    void layout(int* kind, double* ms) const {
+        *kind = zb_ ? 2 : (xt_ ? 1 : 0);
+        for (int i = 0; i < 4; ++i) ms[i] = layout_ms_[i];
     }
+    // ky block of the state and of the spectral side of the pair fields
+    int ky_block() const { return kb_; }
 
     // pass kernels: 0 (inverse z) runs only in prepare_pairs, 1-4 per stage
     static constexpr int kPasses = 5;
@@ -266,6 +274,7 @@ public:
         a.tw = tw_.get();
         a.ms_u = su;
         a.ms_w = sw;
+        a.kb = kb_;
         return a;
     }
 
@@ -297,6 +306,7 @@ public:
         a.tw = tw_.get();
         a.ms_u = su;
         a.ms_w = sw;
+        a.kb = kb_;
         return a;
     }
 
@@ -332,22 +342,26 @@ public:
         g += first * su;
         // transposed layout: the inverse y pass moves the velocity pairs from
         // W (line layout) to X (transposed), the x pass works in X, the
-        // forward y pass moves the product pairs back to W
-        cx<T>* X = xt_ ? X_.get() + first * sw : nullptr;
+        // forward y pass moves the product pairs back to W; ky-blocked
+        // layout: the same moves between W (ky-blocked) and X (line layout)
+        cx<T>* X = xt_ || zb_ ? X_.get() + first * sw : nullptr;
+        const int zv = zb_ ? bvar_[3] : var_[3];
         switch (k) {
             case 0:
-                pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, var_[3], s);
+                pt_->pinv_z(inv_z_args(u, W), PZ_RHS, n, H, members, zv, s);
                 break;
             case 1:
                 if (xt_)
                     pt_->plines_t(W, X, 3, M2, sw, members, true, tw_.get(), xvar_[0], s);
+                else if (zb_)
+                    pt_->plines_b(W, X, 3, M2, sw, members, true, tw_.get(), bvar_[0], s);
                 else
                     pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
             case 2: {
-                PairXArgs<T> a = x_args(xt_ ? X : W);
+                PairXArgs<T> a = x_args(X ? X : W);
                 a.vmax2 = vmax2;
-                const int xv = xt_ ? xvar_[1] : var_[1];
+                const int xv = cur_var(1);
                 if (vmax2)
                     pt_->px_products_vmax(a, xv, xt_, s);
                 else
@@ -357,12 +371,14 @@ public:
             case 3:
                 if (xt_)
                     pt_->plines_t(X, W, 6, M2, sw, members, false, tw_.get(), xvar_[2], s);
+                else if (zb_)
+                    pt_->plines_b(X, W, 6, M2, sw, members, false, tw_.get(), bvar_[2], s);
                 else
                     pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
                 break;
             default: {
                 const PairFwdZArgs<T> a = fwd_z_args(u, g, W, first, stage);
-                pt_->pfwd_z_rk(a, inv_z_args(u, W), n, H, members, var_[3], s);
+                pt_->pfwd_z_rk(a, inv_z_args(u, W), n, H, members, zv, s);
                 break;
             }
         }
@@ -392,7 +408,7 @@ public:
     // dt either (launch_step_mid); the tail is the rest of the step, from
     // stage 0's forward z and update on. One solve per group, register-direct
     // x pass only.
-    bool x_reduces_velocity() const { return count == 1 && (xt_ || var_[1] != kXLdsVariant); }
+    bool x_reduces_velocity() const { return count == 1 && (xt_ || cur_var(1) != kXLdsVariant); }
     void launch_step_head(unsigned long long* vmax2, hipStream_t s) const {
         k_zero_vmax<<<1, kVmaxSlots, 0, s>>>(vmax2);
         launch_pass(1, 0, U_.get(), G_.get(), 0, 1, s);
@@ -419,6 +435,10 @@ public:
         a.n = n;
         a.phase = phase_.get();
         a.tw = tw_.get();
+        a.kb = kb_;
+        // ky-blocked layout: the y pass moves the pairs into the back half of
+        // X (line layout), where the x pass reads them
+        cx<T>* const XY = zb_ ? X_.get() + m * sw + 3 * M2 : W;
         PairInvXArgs<T> x{};
         x.ny = n;
         x.nz = n;
@@ -426,20 +446,26 @@ public:
         x.tw = tw_.get();
         if (mode == PZ_DIAG) {
             pt_->pinv_z(a, PZ_DIAG, n, H, 1, 0, s);
-            pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
+            if (zb_)
+                pt_->plines_b(W, XY, 3, M2, 0, 1, true, tw_.get(), bvar_[0], s);
+            else
+                pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             for (int c = 0; c < 3; ++c) {
-                x.ab[c] = W + c * M2;
+                x.ab[c] = XY + c * M2;
                 x.oa[c] = P + c * npts;
                 x.ob[c] = P + (3 + c) * npts;
             }
             x.npairs = 3;
         } else {
             pt_->pinv_z(a, PZ_VEL, n, H, 1, 0, s);
-            pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
-            x.ab[0] = W;
+            if (zb_)
+                pt_->plines_b(W, XY, 2, M2, 0, 1, true, tw_.get(), bvar_[0], s);
+            else
+                pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
+            x.ab[0] = XY;
             x.oa[0] = P;
             x.ob[0] = P + npts;
-            x.ab[1] = W + M2;
+            x.ab[1] = XY + M2;
             x.oa[1] = P + 2 * npts;
             x.ob[1] = nullptr;
             x.npairs = 2;
@@ -455,7 +481,7 @@ public:
         k_reduce_phys<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(P, P + npts, P + 2 * npts, P + 3 * npts,
                                                                    P + 4 * npts, P + 5 * npts, npts, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 3, 0b010, result_.get());
-        k_reduce_div3<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(U_.get() + m * su, n, partial_.get());
+        k_reduce_div3<T><<<kReduceBlocks, kReduceThreads, 0, s>>>(U_.get() + m * su, n, kb_, partial_.get());
         k_reduce_final<<<1, kReduceThreads, 0, s>>>(partial_.get(), kReduceBlocks, 1, 0b1, result_.get() + 3);
         double r[4];
         hip_check(hipMemcpyAsync(r, result_.get(), sizeof(r), hipMemcpyDeviceToHost, s), "hipMemcpy diagnostics");
@@ -502,10 +528,7 @@ public:
         std::vector<cx<T>> tmp = download_state(m, s);
         for (std::int64_t i = 0; i < modes; ++i)
             tmp[3 * i + c] = {static_cast<T>(host[2 * i]), static_cast<T>(host[2 * i + 1])};
-        hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
-                  "hipMemcpy set_uhat");
-        hip_check(hipStreamSynchronize(s), "set_uhat");
-        pairs_valid_ = false;
+        upload_state(m, tmp, s);
     }
 
     // all three components of member m at once
@@ -514,10 +537,7 @@ public:
         for (std::int64_t i = 0; i < modes; ++i)
             for (int c = 0; c < 3; ++c)
                 tmp[3 * i + c] = {static_cast<T>(u[c][2 * i]), static_cast<T>(u[c][2 * i + 1])};
-        hip_check(hipMemcpyAsync(U_.get() + m * su, tmp.data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
-                  "hipMemcpy set_state");
-        hip_check(hipStreamSynchronize(s), "set_state");
-        pairs_valid_ = false;
+        upload_state(m, tmp, s);
     }
 
     // impl.hpp:796-817 for member m: TG field at cell centres, 3 forward
@@ -544,6 +564,7 @@ public:
         fz.ny = n;
         fz.n = n;
         fz.tw = tw_.get();
+        fz.kb = kb_;
         pt_->fwd_z(fz, FZ_PROJECT, n, H, s);
         hip_check(hipGetLastError(), "initial condition launch");
         pairs_valid_ = false;
@@ -603,29 +624,30 @@ private:
     // the process; SPH_AUTOTUNE=0 keeps variant 0 everywhere,
     // SPH_TILE_VARIANT=v forces variant v wherever it exists.
     struct Choice {
-        int var[kTuned], xvar[kTuned];
-        double ms[kTuned][kMaxVariants], xms[kTuned][kMaxVariants], layout_ms[2];
-        bool xt;
+        int var[kTuned], xvar[kTuned], bvar[kTuned];
+        double ms[kTuned][kMaxVariants], xms[kTuned][kMaxVariants], bms[kTuned][kMaxVariants], layout_ms[4];
+        int layout;  // 0 line, 1 transposed, 2 ky-blocked
     };
     void apply(const Choice& c) {
         for (int p = 0; p < kTuned; ++p) {
             var_[p] = c.var[p];
             xvar_[p] = c.xvar[p];
+            bvar_[p] = c.bvar[p];
             for (int v = 0; v < kMaxVariants; ++v) {
                 tune_ms_[p][v] = c.ms[p][v];
                 xtune_ms_[p][v] = c.xms[p][v];
+                btune_ms_[p][v] = c.bms[p][v];
             }
         }
-        layout_ms_[0] = c.layout_ms[0];
-        layout_ms_[1] = c.layout_ms[1];
-        use_transposed(c.xt);
+        for (int i = 0; i < 4; ++i) layout_ms_[i] = c.layout_ms[i];
+        use_layout(c.layout);
     }
     // The autotuner's choices across processes: with SPH_TUNE_CACHE set to a
     // file path, a choice is looked up there by (library build, device,
     // precision, n, members, forced layout) before timing, and a new one is
     // appended after it. One line per choice:
-    //   sph-tune-1 <build> <device> <bytes> <n> <members> <layout> <xt>
-    //     <var x4> <xvar x4> <layout ms x2> <ms x16> <xms x16>
+    //   sph-tune-2 <build> <device> <bytes> <n> <members> <forced layout> <layout>
+    //     <var x4> <xvar x4> <bvar x4> <layout ms x4> <ms x16> <xms x16> <bms x16>
     // (the last matching line wins). Entries whose variants do not exist in
     // this build or whose layout does not fit are ignored.
     static std::string tune_key(int forced_layout, int n_, int members) {
@@ -636,13 +658,16 @@ private:
             device = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
         for (char& ch : device)
             if (ch == ' ') ch = '_';
-        return std::string("sph-tune-1 ") + SPH_BUILD_ID + " " + device + " " + std::to_string(sizeof(T)) + " " +
+        return std::string("sph-tune-2 ") + SPH_BUILD_ID + " " + device + " " + std::to_string(sizeof(T)) + " " +
                std::to_string(n_) + " " + std::to_string(members) + " " + std::to_string(forced_layout);
     }
-    bool choice_valid(const Choice& c, bool xt_allowed) const {
-        if (c.xt && !xt_allowed) return false;
+    bool choice_valid(const Choice& c, bool xt_allowed, bool zb_allowed) const {
+        if (c.layout < 0 || c.layout > 2) return false;
+        if (c.layout == 1 && !xt_allowed) return false;
+        if (c.layout == 2 && !zb_allowed) return false;
         for (int p = 0; p < kTuned; ++p) {
             if (c.var[p] < 0 || c.var[p] >= pt_->variants[p]) return false;
+            if (c.bvar[p] < 0 || c.bvar[p] >= pt_->variants[p]) return false;
             if (c.xvar[p] < 0 || c.xvar[p] >= kMaxVariants) return false;
         }
         return true;
@@ -666,7 +691,7 @@ private:
             };
             double v = 0.0;
             bool ok = next(v);
-            c.xt = v != 0.0;
+            c.layout = static_cast<int>(v);
             for (int p = 0; ok && p < kTuned; ++p) {
                 ok = next(v);
                 c.var[p] = static_cast<int>(v);
@@ -675,11 +700,17 @@ private:
                 ok = next(v);
                 c.xvar[p] = static_cast<int>(v);
             }
-            for (int i = 0; ok && i < 2; ++i) ok = next(c.layout_ms[i]);
+            for (int p = 0; ok && p < kTuned; ++p) {
+                ok = next(v);
+                c.bvar[p] = static_cast<int>(v);
+            }
+            for (int i = 0; ok && i < 4; ++i) ok = next(c.layout_ms[i]);
             for (int p = 0; ok && p < kTuned; ++p)
                 for (int i = 0; ok && i < kMaxVariants; ++i) ok = next(c.ms[p][i]);
             for (int p = 0; ok && p < kTuned; ++p)
                 for (int i = 0; ok && i < kMaxVariants; ++i) ok = next(c.xms[p][i]);
+            for (int p = 0; ok && p < kTuned; ++p)
+                for (int i = 0; ok && i < kMaxVariants; ++i) ok = next(c.bms[p][i]);
             if (ok) {
                 out = c;
                 found = true;
@@ -689,20 +720,22 @@ private:
         return found;
     }
     static void append_tune_file(const char* path, const std::string& key, const Choice& c) {
-        std::string line = key + " " + (c.xt ? "1" : "0");
+        std::string line = key + " " + std::to_string(c.layout);
         char buf[64];
         for (int p = 0; p < kTuned; ++p) line += " " + std::to_string(c.var[p]);
         for (int p = 0; p < kTuned; ++p) line += " " + std::to_string(c.xvar[p]);
+        for (int p = 0; p < kTuned; ++p) line += " " + std::to_string(c.bvar[p]);
         auto add = [&](double v) {
             std::snprintf(buf, sizeof(buf), " %.6g", v);
             line += buf;
         };
-        add(c.layout_ms[0]);
-        add(c.layout_ms[1]);
+        for (int i = 0; i < 4; ++i) add(c.layout_ms[i]);
         for (int p = 0; p < kTuned; ++p)
             for (int i = 0; i < kMaxVariants; ++i) add(c.ms[p][i]);
         for (int p = 0; p < kTuned; ++p)
             for (int i = 0; i < kMaxVariants; ++i) add(c.xms[p][i]);
+        for (int p = 0; p < kTuned; ++p)
+            for (int i = 0; i < kMaxVariants; ++i) add(c.bms[p][i]);
         line += "\n";
         // one write in append mode: concurrent processes add whole lines
         if (std::FILE* f = std::fopen(path, "a")) {
@@ -711,22 +744,26 @@ private:
         }
     }
     void autotune() {
-        // SPH_LAYOUT=0 / 1 forces the line / transposed work layout (the
-        // latter where the length supports it)
+        // SPH_LAYOUT=0 / 1 / 2 forces the line / transposed / ky-blocked work
+        // layout (the latter two where the length supports them and the
+        // second work buffer fits)
         const char* lay = std::getenv("SPH_LAYOUT");
-        const int forced_layout = lay && (lay[0] == '0' || lay[0] == '1') && lay[1] == '\0' ? lay[0] - '0' : -1;
-        const bool xt_allowed = pt_->xt && forced_layout != 0 && transposed_fits();
+        const int forced_layout = lay && lay[0] >= '0' && lay[0] <= '2' && lay[1] == '\0' ? lay[0] - '0' : -1;
+        const bool fits = (pt_->xt || pt_->zb) && forced_layout != 0 && second_buffer_fits();
+        const bool xt_allowed = pt_->xt && fits && forced_layout != 2;
+        const bool zb_allowed = pt_->zb && fits && forced_layout != 1;
         const char* force = std::getenv("SPH_TILE_VARIANT");
         if (force && force[0] >= '0' && force[0] < '0' + kMaxVariants && force[1] == '\0') {
             const int f = force[0] - '0';
             for (int p = 0; p < kTuned; ++p) {
                 var_[p] = std::min(f, pt_->variants[p] - 1);
+                bvar_[p] = p == 0 || p == 2 ? std::min(f, 2) : var_[p];
                 // transposed layout: y passes 0..2; x pass 0, 1 and the
                 // alternative radix order (no LDS-staged form)
                 const bool x_alt = f == kXOrderVariant && pt_->variants[1] > kXOrderVariant;
                 xvar_[p] = p == 3 ? 0 : (p == 1 ? (x_alt ? f : std::min(f, 1)) : std::min(f, 2));
             }
-            use_transposed(forced_layout == 1 && xt_allowed);
+            use_layout(forced_layout == 1 && xt_allowed ? 1 : (forced_layout == 2 && zb_allowed ? 2 : 0));
             return;
         }
         static std::mutex mu;
@@ -742,14 +779,14 @@ private:
         }
         const char* env = std::getenv("SPH_AUTOTUNE");
         if (env && env[0] == '0') {
-            use_transposed(forced_layout == 1 && xt_allowed);
+            use_layout(forced_layout == 1 && xt_allowed ? 1 : (forced_layout == 2 && zb_allowed ? 2 : 0));
             return;
         }
         const char* tune_file = std::getenv("SPH_TUNE_CACHE");
         const std::string file_key = tune_file && tune_file[0] ? tune_key(forced_layout, n, count) : std::string();
         if (!file_key.empty()) {
             Choice c{};
-            if (load_tune_file(tune_file, file_key, c) && choice_valid(c, xt_allowed)) {
+            if (load_tune_file(tune_file, file_key, c) && choice_valid(c, xt_allowed, zb_allowed)) {
                 apply(c);
                 std::lock_guard<std::mutex> lock(mu);
                 cache.emplace(key, c);
@@ -764,7 +801,8 @@ private:
         const int pass_of[kTuned] = {1, 2, 3, 4};
         // every variant of the tuned passes in the current layout (nvar[p]
         // of pass p; 0: not tuned here); returns the summed time of the
-        // fastest of passes 1-3, the ones the layouts differ in
+        // fastest of passes 1-3, the ones the line and transposed layouts
+        // differ in (the forward-z pass's fastest is in ms[3][var[3]])
         auto tune_layout = [&](int (&var)[kTuned], double (&ms)[kTuned][kMaxVariants], const int* nvar,
                                bool transposed) {
             double total = 0.0;
@@ -790,16 +828,35 @@ private:
             }
             return total;
         };
-        use_transposed(false);
-        layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants, false);
-        bool keep_xt = false;
+        // the line layout is timed unless another one is forced (its
+        // forward-z pass also serves the transposed layout)
+        int keep = 0;
+        double best_other = 0.0;
+        use_layout(0);
+        if (forced_layout != 1 && forced_layout != 2) {
+            layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants, false);
+            best_other = layout_ms_[0];
+        } else {
+            const int nz[kTuned] = {0, 0, 0, pt_->variants[3]};
+            tune_layout(var_, tune_ms_, nz, false);
+        }
         if (xt_allowed) {
-            use_transposed(true);
+            use_layout(1);
             const int nx[kTuned] = {3, pt_->variants[1], 3, 0};
             layout_ms_[1] = tune_layout(xvar_, xtune_ms_, nx, true);
-            keep_xt = forced_layout == 1 || layout_ms_[1] < layout_ms_[0];
+            if (forced_layout == 1 || layout_ms_[1] < layout_ms_[0]) {
+                keep = 1;
+                best_other = layout_ms_[1];
+            }
         }
-        use_transposed(keep_xt);
+        layout_ms_[3] = best_other + tune_ms_[3][var_[3]];
+        if (zb_allowed) {
+            use_layout(2);
+            const int nb[kTuned] = {3, pt_->variants[1], 3, pt_->variants[3]};
+            layout_ms_[2] = tune_layout(bvar_, btune_ms_, nb, false) + btune_ms_[3][bvar_[3]];
+            if (forced_layout == 2 || layout_ms_[2] < layout_ms_[3]) keep = 2;
+        }
+        use_layout(keep);
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         // the timing launches left transforms of zeros in W: clear it again
@@ -809,39 +866,78 @@ private:
         for (int p = 0; p < kTuned; ++p) {
             c.var[p] = var_[p];
             c.xvar[p] = xvar_[p];
+            c.bvar[p] = bvar_[p];
             for (int v = 0; v < kMaxVariants; ++v) {
                 c.ms[p][v] = tune_ms_[p][v];
                 c.xms[p][v] = xtune_ms_[p][v];
+                c.bms[p][v] = btune_ms_[p][v];
             }
         }
-        c.layout_ms[0] = layout_ms_[0];
-        c.layout_ms[1] = layout_ms_[1];
-        c.xt = keep_xt;
+        for (int i = 0; i < 4; ++i) c.layout_ms[i] = layout_ms_[i];
+        c.layout = keep;
         if (!file_key.empty()) append_tune_file(tune_file, file_key, c);
         std::lock_guard<std::mutex> lock(mu);
         cache.emplace(key, c);
     }
 
-    // The transposed layout needs a second work buffer of 12C per member:
-    // taken only with room to spare on the device.
-    bool transposed_fits() const {
+    // The transposed and the ky-blocked layouts need a second work buffer of
+    // 12C per member: taken only with room to spare on the device.
+    bool second_buffer_fits() const {
         std::size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
         const std::size_t need = static_cast<std::size_t>(count) * sw * sizeof(cx<T>);
         return need + (std::size_t(4) << 30) <= free_b;
     }
-    void use_transposed(bool on) {
-        xt_ = on;
-        if (on && !X_.get()) X_.resize(static_cast<std::size_t>(count) * sw);
-        if (!on) X_.release();
+    // 0: line layout; 1: transposed [z][y][kx] between the y passes and the
+    // x pass; 2: ky-blocked state and spectral pair fields (z tiles are
+    // contiguous blocks), line layout in X for the x pass. Only while the
+    // state holds no data (construction), since the state's layout changes.
+    void use_layout(int kind) {
+        xt_ = kind == 1;
+        zb_ = kind == 2;
+        kb_ = zb_ ? pt_->zb_kb : n;
+        if (kind != 0 && !X_.get()) X_.resize(static_cast<std::size_t>(count) * sw);
+        if (kind == 0) X_.release();
     }
 
+    // Member m's state in the plain mode order ((kx*n + kz)*n + ky, three
+    // components per mode), whatever the device layout.
     std::vector<cx<T>> download_state(int m, hipStream_t s) const {
         std::vector<cx<T>> tmp(static_cast<std::size_t>(su));
         hip_check(hipMemcpyAsync(tmp.data(), U_.get() + m * su, su * sizeof(cx<T>), hipMemcpyDeviceToHost, s),
                   "hipMemcpy get_uhat");
         hip_check(hipStreamSynchronize(s), "get_uhat");
-        return tmp;
+        if (kb_ == n) return tmp;
+        std::vector<cx<T>> out(tmp.size());
+        for_each_mode_block([&](std::int64_t plain, std::int64_t dev) {
+            for (int c = 0; c < 3; ++c) out[3 * plain + c] = tmp[3 * dev + c];
+        });
+        return out;
+    }
+    void upload_state(int m, const std::vector<cx<T>>& plain_order, hipStream_t s) {
+        const std::vector<cx<T>>* src = &plain_order;
+        std::vector<cx<T>> dev_order;
+        if (kb_ != n) {
+            dev_order.resize(plain_order.size());
+            for_each_mode_block([&](std::int64_t plain, std::int64_t dev) {
+                for (int c = 0; c < 3; ++c) dev_order[3 * dev + c] = plain_order[3 * plain + c];
+            });
+            src = &dev_order;
+        }
+        hip_check(hipMemcpyAsync(U_.get() + m * su, src->data(), su * sizeof(cx<T>), hipMemcpyHostToDevice, s),
+                  "hipMemcpy set_uhat");
+        hip_check(hipStreamSynchronize(s), "set_uhat");
+        pairs_valid_ = false;
+    }
+    // f(plain mode index, device mode index) over all modes of one member
+    // (ky blocks of kb_: device index kx*n*n + (ky/kb)*n*kb + kz*kb + ky%kb)
+    template <typename F>
+    void for_each_mode_block(F&& f) const {
+        const std::int64_t nn = static_cast<std::int64_t>(n) * n;
+        for (std::int64_t kx = 0; kx < H; ++kx)
+            for (std::int64_t kz = 0; kz < n; ++kz)
+                for (std::int64_t ky = 0; ky < n; ++ky)
+                    f((kx * n + kz) * n + ky, kx * nn + (ky / kb_) * n * kb_ + kz * kb_ + ky % kb_);
     }
 
     const PassTable<T>* pt_ = nullptr;
@@ -854,12 +950,17 @@ private:
     int var_[kTuned] = {0, 0, 0};  // variant of each tuned pass (line layout)
     double tune_ms_[kTuned][kMaxVariants] = {};
     // transposed work layout (k_plines_t, XT x pass): in use, its second work
-    // buffer, variants and times; layout_ms_: passes 1-3 in each layout
+    // buffer, variants and times; ky-blocked layout (k_plines_b): in use,
+    // the ky block, variants and times; layout_ms_ as reported by layout()
     bool xt_ = false;
     DeviceArray<cx<T>> X_;
     int xvar_[kTuned] = {0, 0, 0};
     double xtune_ms_[kTuned][kMaxVariants] = {};
-    double layout_ms_[2] = {0.0, 0.0};
+    bool zb_ = false;
+    int kb_ = 0;
+    int bvar_[kTuned] = {0, 0, 0};
+    double btune_ms_[kTuned][kMaxVariants] = {};
+    double layout_ms_[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 // Per-step GPU time from event pairs, resolved lazily.
@@ -955,7 +1056,7 @@ public:
     virtual void tuning(int* variant, int* lines, double* ms) const = 0;
     virtual void tuned_pass(int pass, int* variant, int* lines, double* ms) const = 0;
     virtual RadixPlan radix_plan_of(int pass) const = 0;
-    virtual void layout(int* transposed, double* ms) const = 0;
+    virtual void layout(int* kind, double* ms) const = 0;
     int n = 0;
     std::int64_t modes = 0;
 };
@@ -1123,7 +1224,7 @@ public:
         g_.tuned_pass(pass, variant, lines, ms);
     }
     RadixPlan radix_plan_of(int pass) const override { return g_.radix_plan_of(pass); }
-    void layout(int* transposed, double* ms) const override { g_.layout(transposed, ms); }
+    void layout(int* kind, double* ms) const override { g_.layout(kind, ms); }
 
     // One adaptive step as the driver takes it (impl.hpp:1363-1378, without
     // the output-time clipping): dt = min(dt_max, cfl * (2 pi / n) / max|u|),

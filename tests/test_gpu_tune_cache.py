@@ -39,7 +39,7 @@ def test_choices_persist_across_processes(tmp_path):
     print("case tune cache: first process", flush=True)
     first = child(env)
     lines = open(path).read().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("sph-tune-1 ")
+    assert len(lines) == 1 and lines[0].startswith("sph-tune-2 ")
     key = " ".join(lines[0].split()[:7])
     print("case tune cache: second process", flush=True)
     second = child(env)
