@@ -54,9 +54,10 @@ def step_bytes(n: int, s: int) -> float:
 
 
 def implemented_step_bytes(n: int, s: int) -> float:
-    """3 x (12C + 18C + 24C + 30C)."""
+    """3 x (12C + 18C + 24C) + (27C + 30C + 27C): the forward-z pass of
+    stage 0 reads no g and that of stage 2 stores none."""
     C = n * n * (n // 2 + 1) * 2 * s
-    return 252.0 * C
+    return 246.0 * C
 
 
 def time_steps(sph, n, precision, steps, warmup, dt):
@@ -410,13 +411,11 @@ def main():
                 "isolated_launch_ms": round(iso_ms, 4) if iso_ms else None,
                 "isolated_frac": round(by / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if iso_ms else None,
                 "algorithmic_bytes": by,
-                "algorithmic_bytes_note": ("the five-pass model's bytes per launch of this pass; for the forward z + "
-                                           "update pass 30C, of which stage 0 needs 27C (no g to read), so over "
-                                           "a step's three launches the mean is 29C: frac_stage_mean_bytes"
+                "algorithmic_bytes_note": ("bytes per launch of this pass, each array read or written once; the "
+                                           "forward z + update pass moves 27C, 30C and 27C in the three stages "
+                                           "(stage 0 reads no g, stage 2 stores none): the mean, 28C"
                                            if PASS_NAMES[dom] == "forward_z_update" else
-                                           "the five-pass model's bytes per launch of this pass"),
-                "frac_stage_mean_bytes": (round(by * 29.0 / 30.0 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                                          if PASS_NAMES[dom] == "forward_z_update" else None),
+                                           "bytes per launch of this pass, each array read or written once"),
                 "step_frac": round(step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "step_frac_implemented_bytes": round(implemented_step_bytes(n, 4) / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                 "passes_in_step_ms": {k: round(instep[k][0], 4) for k in PASS_NAMES[1:]},

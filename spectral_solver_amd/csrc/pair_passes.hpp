@@ -47,12 +47,16 @@ struct PairInvZArgs {
 
 template <typename T>
 struct PairInvXArgs {
-    const cx<T>* ab[3];  // pair fields (A, B) in mixed space [kx][z][y][2]
+    const cx<T>* ab[3];  // pair fields (A, B) in mixed space, rows krow / zrow apart (below)
     T* oa[3];
     T* ob[3];  // may be null (B discarded)
     int npairs;
     int ny;
     int nz;
+    // pairs between kx rows and between z planes of the mixed-space fields:
+    // [kx][z][y] (krow = nz*ny, zrow = ny) or [z][kx][y] (krow = ny, zrow =
+    // (n/2+1)*ny)
+    long long krow, zrow;
     T scale;
     const cx<T>* tw;
     long long ms_w;
@@ -439,13 +443,14 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
 // ---------------------------------------------------------------------------
 // y lines between the line layout of the x pass and the ky-blocked spectral
 // layout of the z passes (out of place):
-//   line layout   [kx][z][y][2]                offset ((kx*n + z)*n + y)*2 + f
+//   line layout   [kx][z][y][2] or [z][kx][y][2]: offset kx*xs_kx + z*xs_z + 2y + f
 //   ky-blocked    [kx][ky/KB][kz][ky%KB][2]    offset (kx*n*n + (ky/KB)*n*KB + kz*KB + ky%KB)*2 + f
 // with KB the default z tile's NL/2, so that a z tile is one contiguous block
 // of each field (at n = 1024 the z passes' 64-byte rows 16 KB apart cost them
 // their streaming rate). A tile is NL/2 pair lines of consecutive z (kz) of
-// one kx: on the line side NL*N contiguous values; on the blocked side, per ky
-// block, the tile's NL/2 rows of KB modes, NL*KB contiguous values. Thread q
+// one kx: on the line side NL/2 lines of 2N contiguous values, xs_z apart; on
+// the blocked side, per ky block, the tile's NL/2 rows of KB modes, NL*KB
+// contiguous values. Thread q
 // of a side takes that side's q-th value in address order. INV: blocked
 // (spectral ky) in, line layout (physical y) out; forward: the other way.
 // Grid (pair lines / (NL/2), 1, field + nfields*member).
@@ -457,7 +462,7 @@ constexpr int zblock_ky() {
 template <typename T, int N, bool INV, int LINES>
 __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     k_plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
-               const cx<T>* tw) {
+               long long xs_kx, long long xs_z, const cx<T>* tw) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, KB = zblock_ky<T, N>();
     constexpr int G = NL / 2;       // pair lines per tile
@@ -469,13 +474,16 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
     __shared__ cx<T> twl[N];
     const int tid = threadIdx.x;
     const TileId tile = xcd_tile_id();
+    // consecutive tiles: consecutive z blocks of one kx, whose rows are
+    // neighbours in every ky block (kx fastest instead, where the line side
+    // is [z][kx][y]: 1024^3 forward y 17.1 -> 21.6 ms)
     const long long g0 = static_cast<long long>(tile.x) * G;  // pair line kx*N + z of the tile's first line
     const int kx = static_cast<int>(g0 / N), z0 = static_cast<int>(g0 % N);
     const int field = tile.z % nfields, member = tile.z / nfields;
     const long long fo = field * field_stride + member * member_stride;
     const cx<T>* src = in + fo;
     cx<T>* dst = out + fo;
-    const long long line0 = g0 * 2 * N;                              // line side: the tile's first value
+    const long long line0 = kx * xs_kx + z0 * xs_z;                 // line side: the tile's first value
     const long long blk0 = (static_cast<long long>(kx) * N * N + static_cast<long long>(z0) * KB) * 2;
     // blocked side: q-th value of the tile in address order -> offset, ky, line
     auto blk = [&](int q, int& ky, int& line) -> long long {
@@ -496,7 +504,7 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
             buf[tile_at<NL, SWZ>(ky, line)] = gld<kPolicyStream>(src + o);
         } else {
             const int g = q / (2 * N), rem = q % (2 * N);
-            buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = gld<kPolicyStream>(src + line0 + q);
+            buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))] = gld<kPolicyStream>(src + line0 + g * xs_z + rem);
         }
     }
     __syncthreads();
@@ -507,7 +515,7 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
         if (TL::ELEMS % NT != 0 && q >= TL::ELEMS) break;
         if constexpr (INV) {
             const int g = q / (2 * N), rem = q % (2 * N);
-            gst<kPolicyStream>(dst + line0 + q, buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
+            gst<kPolicyStream>(dst + line0 + g * xs_z + rem, buf[tile_at<NL, SWZ>(rem >> 1, 2 * g + (rem & 1))]);
         } else {
             int ky, line;
             const long long o = blk(q, ky, line);
@@ -526,7 +534,7 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
 // (Hermitian extension of both members; Re rule on kx = 0 and N/2).
 template <typename T, int N, int LINES>
 __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, int tid, int y0, int z, int ny,
-                                                  int nz) {
+                                                  long long krow, long long zrow) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
     constexpr int LOADS = NL * (N / 2 + 1);
@@ -538,7 +546,7 @@ __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, i
             const int l = idx % NL, k = idx / NL;
             // lines past the last one read a valid row and discard it
             const bool ok = y0 + l < ny;
-            const size_t off = (static_cast<size_t>(k) * nz + z) * ny + (ok ? y0 + l : ny - 1);
+            const size_t off = k * krow + z * zrow + (ok ? y0 + l : ny - 1);
             const cx2<T> q = reinterpret_cast<const cx2<T>*>(AB)[off];
             cx<T> va = ok ? q.a : czero<T>(), vb = ok ? q.b : czero<T>();
             if (k == 0 || k == N / 2) {
@@ -557,7 +565,7 @@ __device__ __forceinline__ void load_pair_x_lines(cx<T>* buf, const cx<T>* AB, i
 template <typename T, int N, int LINES, int E>
 __device__ __forceinline__ void products_forward_x(const T (&u)[3][E], const T (&s)[3][E], cx<T>* buf,
                                                    const cx<T>* twl, cx<T>* const (&out)[6], long long zw, int tid,
-                                                   int y0, int z, int ny, int nz) {
+                                                   int y0, int z, int ny, long long krow, long long zrow) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, S = TL::STRIDE;
     static_assert(E == TL::E, "register slots must match the tile");
@@ -584,7 +592,7 @@ __device__ __forceinline__ void products_forward_x(const T (&u)[3][E], const T (
                 if (y0 + l < ny) {
                     const cx<T> zk = buf[k * S + l];
                     const cx<T> zm = buf[((N - k) % N) * S + l];
-                    const size_t off = (static_cast<size_t>(k) * nz + z) * ny + y0 + l;
+                    const size_t off = k * krow + z * zrow + y0 + l;
                     cx2<T> q;
                     q.a = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
                     q.b = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
@@ -612,7 +620,7 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
     load_twiddles<T, N, NT>(twl, a.tw, tid);
 #pragma unroll 1
     for (int p = 0; p < a.npairs; ++p) {
-        load_pair_x_lines<T, N, NL>(buf, pick(a.ab, p) + zw, tid, y0, z, a.ny, a.nz);
+        load_pair_x_lines<T, N, NL>(buf, pick(a.ab, p) + zw, tid, y0, z, a.ny, a.krow, a.zrow);
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
         T* OA = pick(a.oa, p) + zp;
@@ -644,10 +652,15 @@ __global__ void __launch_bounds__((Tile<T, N>::NT)) k_pinv_x(PairInvXArgs<T> a) 
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PairXArgs {
-    const cx<T>* ab[3];  // (u_hat_c, (u_hat phi)_c) after the z and y transforms, [kx][z][y][2]
-    cx<T>* out[6];       // product pairs, [kx][z][y][2]
+    const cx<T>* ab[3];  // (u_hat_c, (u_hat phi)_c) after the z and y transforms
+    cx<T>* out[6];       // product pairs, same layout
     int ny;
     int nz;
+    // pairs between kx rows and between z planes: [kx][z][y][2] (krow =
+    // nz*ny, zrow = ny) or [z][kx][y][2] (krow = ny, zrow = (n/2+1)*ny; an x
+    // tile's rows then lie within one z plane's n/2+1 rows); unused in the
+    // transposed layout (XT)
+    long long krow, zrow;
     T scale;
     const cx<T>* tw;
     long long ms_w;
@@ -718,7 +731,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     const int z = tile.y;
     const bool ok = y0 + l < a.ny;
     const long long zw = tile.z * a.ms_w;
-    const uint32_t row = static_cast<uint32_t>(a.nz) * a.ny;  // cx2 elements between k rows
+    const uint32_t row = static_cast<uint32_t>(a.krow);  // cx2 elements between k rows
     // per-slot 32-bit addressing fits when STEP rows of a cubic grid span
     // less than 2 GiB (everything but FP64 at N = 1024); the launcher only
     // runs this form with ny * nz <= N * N
@@ -747,7 +760,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
     constexpr bool DMA = kXPrefetch && sizeof(T) == 4 && PP && !XT && kNarrow && PRE_INS * 64 * 2 <= Z::LDS;
     cx2<T>* const pre = reinterpret_cast<cx2<T>*>(tile_b);
     auto prefetch = [&](int c) {
-        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + static_cast<size_t>(z) * a.ny + y0;
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + static_cast<size_t>(z) * a.zrow + y0;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<cx2<T>*>(AB), 0, static_cast<int>(((H - 1) * row + NL) * sizeof(cx2<T>)), 0x00020000);
         // a fixed, unrolled count per wave (the remainder under a uniform
@@ -779,7 +792,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
             __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
             __syncthreads();
         }
-        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + (XT ? xt_block : static_cast<size_t>(z) * a.ny + y0);
+        const cx2<T>* AB = reinterpret_cast<const cx2<T>*>(a.ab[c] + zw) + (XT ? xt_block : static_cast<size_t>(z) * a.zrow + y0);
         // XT: one resource for the tile's block; lanes past the last line read zeros
         const __amdgpu_buffer_rsrc_t rsx = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<cx2<T>*>(AB), 0, static_cast<int>(NL * H * sizeof(cx2<T>)), 0x00020000);
@@ -881,7 +894,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
             const int j = Z::j_mirror(tid);
             const bool self0 = j == 0, selfh = j == Z::STEP / 2;
             cx2<T>* o = reinterpret_cast<cx2<T>*>(a.out[p] + zw) +
-                        (XT ? xt_block + static_cast<size_t>(l) * H : static_cast<size_t>(z) * a.ny + y0 + l);
+                        (XT ? xt_block + static_cast<size_t>(l) * H : static_cast<size_t>(z) * a.zrow + y0 + l);
             const size_t ostep = XT ? 1 : row;  // between rows k
 #pragma unroll
             for (int r = 0; r <= HR; ++r) {
@@ -924,7 +937,7 @@ __device__ __forceinline__ void px_products_rd_tile(const PairXArgs<T>& a, TileI
                     if constexpr (XT)
                         gst2<XPOL>(o + xt_block + static_cast<size_t>(l2) * H + k, q);
                     else
-                        gst2<XPOL>(o + k * row + static_cast<size_t>(z) * a.ny + y0 + l2, q);
+                        gst2<XPOL>(o + k * row + static_cast<size_t>(z) * a.zrow + y0 + l2, q);
                 }
             }
         }
@@ -968,7 +981,7 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
     T u[3][E], s[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        load_pair_x_lines<T, N, NL>(buf, a.ab[c] + zw, tid, y0, z, a.ny, a.nz);
+        load_pair_x_lines<T, N, NL>(buf, a.ab[c] + zw, tid, y0, z, a.ny, a.krow, a.zrow);
         __syncthreads();
         fft_tile<T, N, NL, NT, true>(buf, twl, tid);
 #pragma unroll
@@ -982,7 +995,7 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT)) k_px_products(PairXAr
         }
         __syncthreads();
     }
-    products_forward_x<T, N, NL, E>(u, s, buf, twl, a.out, zw, tid, y0, z, a.ny, a.nz);
+    products_forward_x<T, N, NL, E>(u, s, buf, twl, a.out, zw, tid, y0, z, a.ny, a.krow, a.zrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -1126,11 +1139,14 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
         }
         project_or_zero(u[h][0], u[h][1], u[h][2], kx, my, mz, k2, keep);
     }
+    // g after the last stage is never read (the next step's first stage sets
+    // g = RHS without reading it, impl.hpp:994-1000): stage 2 does not store it
+    const bool keep_g = a.stage != 2;
 #pragma unroll
     for (int h = 0; h < HS; ++h)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            G.st(off[h] + c * CB, g[h][c]);
+            if (keep_g) G.st(off[h] + c * CB, g[h][c]);
             U.st(off[h] + c * CB, u[h][c]);
         }
     if (stash) {

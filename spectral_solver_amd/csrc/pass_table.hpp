@@ -74,8 +74,11 @@ struct PassTable {
     // variants 0..2: the line layout's y-tile widths)
     bool zb;
     int zb_kb;
+    // (xs_kx, xs_z: complex values between kx lines and between z lines of
+    // the line side)
     void (*plines_b)(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
-                     int members, bool inverse, const cx<T>* tw, int variant, hipStream_t);
+                     long long xs_kx, long long xs_z, int members, bool inverse, const cx<T>* tw, int variant,
+                     hipStream_t);
 };
 
 // nullptr when n is not a compiled length.
@@ -239,24 +242,28 @@ struct Launch {
     static constexpr bool kZB = N >= 256;
     template <int L>
     static void plines_b_l(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
-                           int members, bool inverse, const cx<T>* tw, hipStream_t s) {
+                           long long xs_kx, long long xs_z, int members, bool inverse, const cx<T>* tw,
+                           hipStream_t s) {
         using TV = Tile<T, N, L>;
         constexpr long long H = N / 2 + 1;
         const dim3 grid(static_cast<unsigned>(H * N / (L / 2)), 1, nfields * members);
         if (inverse)
-            k_plines_b<T, N, true, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+            k_plines_b<T, N, true, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, xs_kx,
+                                                              xs_z, tw);
         else
-            k_plines_b<T, N, false, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, tw);
+            k_plines_b<T, N, false, L><<<grid, TV::NT, 0, s>>>(in, out, nfields, field_stride, member_stride, xs_kx,
+                                                               xs_z, tw);
     }
     static void plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
-                         int members, bool inverse, const cx<T>* tw, int variant, hipStream_t s) {
+                         long long xs_kx, long long xs_z, int members, bool inverse, const cx<T>* tw, int variant,
+                         hipStream_t s) {
         if constexpr (kZB) {
             if (variant == 2)
-                plines_b_l<L2>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+                plines_b_l<L2>(in, out, nfields, field_stride, member_stride, xs_kx, xs_z, members, inverse, tw, s);
             else if (variant == 1)
-                plines_b_l<L1>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+                plines_b_l<L1>(in, out, nfields, field_stride, member_stride, xs_kx, xs_z, members, inverse, tw, s);
             else
-                plines_b_l<L0>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
+                plines_b_l<L0>(in, out, nfields, field_stride, member_stride, xs_kx, xs_z, members, inverse, tw, s);
         }
     }
     static void px_products(const PairXArgs<T>& a, int ny, int nz, int members, int variant, bool xt,

@@ -237,6 +237,12 @@ public:
     }
     // ky block of the state and of the spectral side of the pair fields
     int ky_block() const { return kb_; }
+    // pairs between kx rows / z planes of the x pass's fields: [kx][z][y] in
+    // the line layout, [z][kx][y] in the ky-blocked one (an x tile's n/2+1
+    // rows then lie in one plane of (n/2+1)*n pairs instead of n/2+1 planes
+    // n*n pairs apart)
+    long long x_krow() const { return xz_ ? n : static_cast<long long>(n) * n; }
+    long long x_zrow() const { return xz_ ? static_cast<long long>(H) * n : n; }
 
     // pass kernels: 0 (inverse z) runs only in prepare_pairs, 1-4 per stage
     static constexpr int kPasses = 5;
@@ -287,6 +293,8 @@ public:
         for (int q = 0; q < 6; ++q) a.out[q] = src + q * M2;
         a.ny = n;
         a.nz = n;
+        a.krow = x_krow();
+        a.zrow = x_zrow();
         a.scale = T(1) / static_cast<T>(npts);
         a.tw = tw_.get();
         a.ms_w = sw;
@@ -354,7 +362,7 @@ public:
                 if (xt_)
                     pt_->plines_t(W, X, 3, M2, sw, members, true, tw_.get(), xvar_[0], s);
                 else if (zb_)
-                    pt_->plines_b(W, X, 3, M2, sw, members, true, tw_.get(), bvar_[0], s);
+                    pt_->plines_b(W, X, 3, M2, sw, 2 * x_krow(), 2 * x_zrow(), members, true, tw_.get(), bvar_[0], s);
                 else
                     pt_->plines(W, 3LL * H * n, sw, members, true, tw_.get(), var_[0], s);
                 break;
@@ -372,7 +380,8 @@ public:
                 if (xt_)
                     pt_->plines_t(X, W, 6, M2, sw, members, false, tw_.get(), xvar_[2], s);
                 else if (zb_)
-                    pt_->plines_b(X, W, 6, M2, sw, members, false, tw_.get(), bvar_[2], s);
+                    pt_->plines_b(X, W, 6, M2, sw, 2 * x_krow(), 2 * x_zrow(), members, false, tw_.get(), bvar_[2],
+                                  s);
                 else
                     pt_->plines(W, 6LL * H * n, sw, members, false, tw_.get(), var_[2], s);
                 break;
@@ -442,12 +451,14 @@ public:
         PairInvXArgs<T> x{};
         x.ny = n;
         x.nz = n;
+        x.krow = x_krow();
+        x.zrow = x_zrow();
         x.scale = T(1) / static_cast<T>(npts);
         x.tw = tw_.get();
         if (mode == PZ_DIAG) {
             pt_->pinv_z(a, PZ_DIAG, n, H, 1, 0, s);
             if (zb_)
-                pt_->plines_b(W, XY, 3, M2, 0, 1, true, tw_.get(), bvar_[0], s);
+                pt_->plines_b(W, XY, 3, M2, 0, 2 * x_krow(), 2 * x_zrow(), 1, true, tw_.get(), bvar_[0], s);
             else
                 pt_->plines(W, 3LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             for (int c = 0; c < 3; ++c) {
@@ -459,7 +470,7 @@ public:
         } else {
             pt_->pinv_z(a, PZ_VEL, n, H, 1, 0, s);
             if (zb_)
-                pt_->plines_b(W, XY, 2, M2, 0, 1, true, tw_.get(), bvar_[0], s);
+                pt_->plines_b(W, XY, 2, M2, 0, 2 * x_krow(), 2 * x_zrow(), 1, true, tw_.get(), bvar_[0], s);
             else
                 pt_->plines(W, 2LL * H * n, 0, 1, true, tw_.get(), var_[0], s);
             x.ab[0] = XY;
@@ -609,10 +620,12 @@ public:
     // once), C = one complex half-spectrum field of every member: inverse z
     // 3C + 6C; inverse y 6C + 6C; x pass 6C + 12C; forward y 12C + 12C;
     // forward z + update + next inverse z 12C + u, g (6C) in, u, g (6C) and
-    // the next stage's velocity pairs (6C) out (its state comes from LDS).
+    // the next stage's velocity pairs (6C) out (its state comes from LDS),
+    // except that stage 0 reads no g and stage 2 stores none: 27C, 30C, 27C,
+    // 28C per launch over a step.
     void pass_bytes(double* bytes) const {
         const double C = static_cast<double>(modes) * 2.0 * sizeof(T) * count;
-        const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 30 * C};
+        const double b5[kPasses] = {9 * C, 12 * C, 18 * C, 24 * C, 28 * C};
         for (int k = 0; k < kPasses; ++k) bytes[k] = b5[k];
     }
 
@@ -896,6 +909,8 @@ private:
         xt_ = kind == 1;
         zb_ = kind == 2;
         kb_ = zb_ ? pt_->zb_kb : n;
+        const char* xz = std::getenv("SPH_X_ZMAJOR");
+        xz_ = zb_ && !(xz && xz[0] == '0');
         if (kind != 0 && !X_.get()) X_.resize(static_cast<std::size_t>(count) * sw);
         if (kind == 0) X_.release();
     }
@@ -957,6 +972,7 @@ private:
     int xvar_[kTuned] = {0, 0, 0};
     double xtune_ms_[kTuned][kMaxVariants] = {};
     bool zb_ = false;
+    bool xz_ = false;  // the x pass's fields [z][kx][y] (ky-blocked layout)
     int kb_ = 0;
     int bvar_[kTuned] = {0, 0, 0};
     double btune_ms_[kTuned][kMaxVariants] = {};

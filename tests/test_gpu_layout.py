@@ -1,9 +1,9 @@
 """The work layouts against each other and against the CPU oracle:
 SPH_LAYOUT=1, the transposed layout (k_plines_t and the x pass on [z][y][kx]
 pair fields), and SPH_LAYOUT=2, the ky-blocked layout (state and spectral pair
-fields in blocks of the z tile's ky count, k_plines_b between them and the
-line layout of the x pass), against SPH_LAYOUT=0, the line layout: the same
-arithmetic on other addresses, so the states agree to rounding, and the FP64
+fields in blocks of the z tile's ky count, k_plines_b between them and the x
+pass's fields in [z][kx][y], or [kx][z][y] with SPH_X_ZMAJOR=0), against
+SPH_LAYOUT=0, the line layout: the same arithmetic on other addresses, so the states agree to rounding, and the FP64
 states agree with the oracle. Also state access and a random state with
 non-Hermitian Nyquist modes in the ky-blocked layout, an ensemble of two
 members in it, the adaptive step's fused velocity maximum in both, and the
@@ -53,6 +53,24 @@ def test_layout_agrees_with_line_layout(monkeypatch, layout, n, precision, varia
     assert err <= tol
     assert abs(da.dissipation - db.dissipation) <= tol * db.dissipation
     assert abs(da.kinetic_energy - db.kinetic_energy) <= tol * db.kinetic_energy
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-14), ("fp32", 1e-6)])
+def test_blocked_layout_with_kx_major_x_fields(monkeypatch, precision, tol):
+    """SPH_X_ZMAJOR=0: the ky-blocked layout with the x pass's fields in
+    [kx][z][y] instead of [z][kx][y] (other strides in the x pass, its
+    diagnostics transforms and the y passes' line side)."""
+    print("case blocked layout, kx-major x fields", precision, flush=True)
+    monkeypatch.setenv("SPH_X_ZMAJOR", "0")
+    a, da = run(monkeypatch, "2", 256, precision, 2, 1e-3)
+    monkeypatch.delenv("SPH_X_ZMAJOR")
+    b, db = run(monkeypatch, "2", 256, precision, 2, 1e-3)
+    c, dc = run(monkeypatch, "0", 256, precision, 2, 1e-3)
+    for x, dx in ((a, da), (b, db)):
+        err = rel(x, c)
+        print(f"  vs line layout: rel L2 {err:.2e}", flush=True)
+        assert err <= tol
+        assert abs(dx.dissipation - dc.dissipation) <= tol * dc.dissipation
 
 
 @pytest.mark.parametrize("layout", ["1", "2"])
