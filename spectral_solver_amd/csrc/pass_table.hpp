@@ -237,9 +237,9 @@ struct Launch {
                 plines_t_l<T0L>(in, out, nfields, field_stride, member_stride, members, inverse, tw, s);
         }
     }
-    // the ky-blocked spectral layout, at the lengths where the z passes'
-    // rows lie far apart
-    static constexpr bool kZB = N >= 256;
+    // the ky-blocked spectral layout, at the lengths whose ky blocks and y
+    // tiles divide them
+    static constexpr bool kZB = N >= 64;
     template <int L>
     static void plines_b_l(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
                            long long xs_kx, long long xs_z, int members, bool inverse, const cx<T>* tw,

@@ -237,6 +237,9 @@ public:
     }
     // ky block of the state and of the spectral side of the pair fields
     int ky_block() const { return kb_; }
+    // the line layout in use (the one-launch-per-pass ensemble kernels know
+    // no other)
+    bool line_layout() const { return !xt_ && !zb_; }
     // pairs between kx rows / z planes of the x pass's fields: [kx][z][y] in
     // the line layout, [z][kx][y] in the ky-blocked one (an x tile's n/2+1
     // rows then lie in one plane of (n/2+1)*n pairs instead of n/2+1 planes
@@ -1435,7 +1438,8 @@ private:
     // directly) and the faster is kept.
     void make_fused() {
         for (std::size_t q = 0; q < groups_.size(); ++q)
-            if (ens_size(groups_[q]->n) && static_cast<int>(fused_.size()) < kEnsGroups) fused_.push_back(q);
+            if (ens_size(groups_[q]->n) && groups_[q]->line_layout() && static_cast<int>(fused_.size()) < kEnsGroups)
+                fused_.push_back(q);
         if (fused_.empty()) return;
         std::sort(fused_.begin(), fused_.end(), [&](std::size_t a, std::size_t b) { return groups_[a]->n > groups_[b]->n; });
         std::vector<EnsGroupArgs<T>> args;
