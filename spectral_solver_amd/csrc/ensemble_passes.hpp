@@ -64,6 +64,12 @@ struct EnsGroupArgs {
     cx<T>* lines;                 // pair fields of member 0 (y passes)
     long long npairlines[2];      // inverse y (3 fields), forward y (6 fields)
     long long member_stride;      // complex values between members' work buffers
+    // ky-blocked layout (Group::use_layout(2)): the y passes move pairs
+    // between lines (ky-blocked) and xlines (the x pass's fields, kx lines
+    // xs_kx and z lines xs_z complex values apart), fields field_stride apart
+    int blocked;
+    cx<T>* xlines;
+    long long xs_kx, xs_z, field_stride;
 };
 
 // The launch: group q owns logical blocks first[q] .. first[q+1]-1, laid out
@@ -97,12 +103,25 @@ __device__ __forceinline__ void ens_tile(const EnsGroupArgs<T>& a, TileId t, int
     load_twiddles<T, N, kEnsThreads>(twl, a.tw, tid);
     if constexpr (PASS == 0) {
         pinv_z_tile<T, N, PZ_RHS, ZL>(a.inv, t, lds, twl, tid);
-    } else if constexpr (PASS == 1) {
-        plines_tile<T, N, true, YL>(a.lines, a.npairlines[0], a.member_stride, t, lds, twl, tid);
+    } else if constexpr (PASS == 1 || PASS == 3) {
+        constexpr bool INV = PASS == 1;
+        if (a.blocked) {
+            if constexpr (N % (YL / 2) == 0 && N % zblock_ky<T, N>() == 0) {
+                // tile t.x of the group's pair lines, field by field; t.y the member
+                const long long per_field = static_cast<long long>(N / 2 + 1) * N / (YL / 2);
+                const int field = static_cast<int>(t.x / per_field);
+                const long long g0 = (t.x % per_field) * (YL / 2);
+                const long long fo = field * a.field_stride + static_cast<long long>(t.y) * a.member_stride;
+                const cx<T>* src = (INV ? a.lines : a.xlines) + fo;
+                cx<T>* dst = (INV ? a.xlines : a.lines) + fo;
+                plines_b_tile<T, N, INV, YL>(src, dst, a.xs_kx, a.xs_z, static_cast<int>(g0 / N),
+                                             static_cast<int>(g0 % N), lds, twl, tid);
+            }
+        } else {
+            plines_tile<T, N, INV, YL>(a.lines, a.npairlines[INV ? 0 : 1], a.member_stride, t, lds, twl, tid);
+        }
     } else if constexpr (PASS == 2) {
         px_products_rd_tile<T, N, ZL>(a.x, t, lds, twl, tid);
-    } else if constexpr (PASS == 3) {
-        plines_tile<T, N, false, YL>(a.lines, a.npairlines[1], a.member_stride, t, lds, twl, tid);
     } else {
         constexpr bool kStash = z_stash_fits<T, N, ZL>();
         pfwd_z_rk_tile<T, N, ZL>(a.fwd[stage], t, lds, twl, tid, kStash ? lds : nullptr);

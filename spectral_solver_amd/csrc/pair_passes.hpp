@@ -459,31 +459,19 @@ template <typename T, int N>
 constexpr int zblock_ky() {
     return ZTile<T, N>::NL / 2;
 }
+// One tile: pair lines z0 .. z0 + NL/2 - 1 of kx, the field and member
+// offsets already applied to src and dst (twl holds the twiddles, buf
+// tile_lds values).
 template <typename T, int N, bool INV, int LINES>
-__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
-    k_plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
-               long long xs_kx, long long xs_z, const cx<T>* tw) {
+__device__ __forceinline__ void plines_b_tile(const cx<T>* src, cx<T>* dst, long long xs_kx, long long xs_z, int kx,
+                                              int z0, cx<T>* buf, const cx<T>* twl, int tid) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, KB = zblock_ky<T, N>();
-    constexpr int G = NL / 2;       // pair lines per tile
-    constexpr int RUN = NL * KB;    // contiguous values per ky block on the blocked side
+    constexpr int RUN = NL * KB;  // contiguous values per ky block on the blocked side
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
-    static_assert(N % G == 0 && N % KB == 0, "tiles must not straddle a kx plane or a ky block");
+    static_assert(N % (NL / 2) == 0 && N % KB == 0, "tiles must not straddle a kx plane or a ky block");
     constexpr bool SWZ = plines_swizzled<T, N>(NL);
-    __shared__ cx<T> buf[tile_lds<NL, SWZ>(N)];
-    __shared__ cx<T> twl[N];
-    const int tid = threadIdx.x;
-    const TileId tile = xcd_tile_id();
-    // consecutive tiles: consecutive z blocks of one kx, whose rows are
-    // neighbours in every ky block (kx fastest instead, where the line side
-    // is [z][kx][y]: 1024^3 forward y 17.1 -> 21.6 ms)
-    const long long g0 = static_cast<long long>(tile.x) * G;  // pair line kx*N + z of the tile's first line
-    const int kx = static_cast<int>(g0 / N), z0 = static_cast<int>(g0 % N);
-    const int field = tile.z % nfields, member = tile.z / nfields;
-    const long long fo = field * field_stride + member * member_stride;
-    const cx<T>* src = in + fo;
-    cx<T>* dst = out + fo;
-    const long long line0 = kx * xs_kx + z0 * xs_z;                 // line side: the tile's first value
+    const long long line0 = kx * xs_kx + z0 * xs_z;  // line side: the tile's first value
     const long long blk0 = (static_cast<long long>(kx) * N * N + static_cast<long long>(z0) * KB) * 2;
     // blocked side: q-th value of the tile in address order -> offset, ky, line
     auto blk = [&](int q, int& ky, int& line) -> long long {
@@ -493,7 +481,6 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
         line = 2 * g + f;
         return blk0 + static_cast<long long>(b) * N * KB * 2 + r;
     };
-    load_twiddles<T, N, NT>(twl, tw, tid);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         const int q = tid + i * NT;
@@ -522,6 +509,27 @@ __global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
             gst<kPolicyStream>(dst + o, buf[tile_at<NL, SWZ>(ky, line)]);
         }
     }
+}
+
+template <typename T, int N, bool INV, int LINES>
+__global__ void __launch_bounds__((Tile<T, N, LINES>::NT))
+    k_plines_b(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
+               long long xs_kx, long long xs_z, const cx<T>* tw) {
+    using TL = Tile<T, N, LINES>;
+    constexpr int G = TL::NL / 2;  // pair lines per tile
+    __shared__ cx<T> buf[tile_lds<TL::NL, plines_swizzled<T, N>(TL::NL)>(N)];
+    __shared__ cx<T> twl[N];
+    const int tid = threadIdx.x;
+    const TileId tile = xcd_tile_id();
+    // consecutive tiles: consecutive z blocks of one kx, whose rows are
+    // neighbours in every ky block (kx fastest instead, where the line side
+    // is [z][kx][y]: 1024^3 forward y 17.1 -> 21.6 ms)
+    const long long g0 = static_cast<long long>(tile.x) * G;  // pair line kx*N + z of the tile's first line
+    const int kx = static_cast<int>(g0 / N), z0 = static_cast<int>(g0 % N);
+    const int field = tile.z % nfields, member = tile.z / nfields;
+    const long long fo = field * field_stride + member * member_stride;
+    load_twiddles<T, N, TL::NT>(twl, tw, tid);
+    plines_b_tile<T, N, INV, LINES>(in + fo, out + fo, xs_kx, xs_z, kx, z0, buf, twl, tid);
 }
 
 // ---------------------------------------------------------------------------

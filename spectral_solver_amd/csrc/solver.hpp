@@ -237,9 +237,9 @@ public:
     }
     // ky block of the state and of the spectral side of the pair fields
     int ky_block() const { return kb_; }
-    // the line layout in use (the one-launch-per-pass ensemble kernels know
-    // no other)
-    bool line_layout() const { return !xt_ && !zb_; }
+    // the transposed layout in use (the one-launch-per-pass ensemble kernels
+    // have no form for it)
+    bool transposed_layout() const { return xt_; }
     // pairs between kx rows / z planes of the x pass's fields: [kx][z][y] in
     // the line layout, [z][kx][y] in the ky-blocked one (an x tile's n/2+1
     // rows then lie in one plane of (n/2+1)*n pairs instead of n/2+1 planes
@@ -330,11 +330,16 @@ public:
         e.tw = tw_.get();
         e.inv = inv_z_args(U_.get(), W);
         for (int st = 0; st < 3; ++st) e.fwd[st] = fwd_z_args(U_.get(), G_.get(), W, 0, st);
-        e.x = x_args(W);
+        e.x = x_args(zb_ ? X_.get() : W);
         e.lines = W;
         e.npairlines[0] = 3LL * H * n;
         e.npairlines[1] = 6LL * H * n;
         e.member_stride = sw;
+        e.blocked = zb_ ? 1 : 0;
+        e.xlines = zb_ ? X_.get() : nullptr;
+        e.xs_kx = 2 * x_krow();
+        e.xs_z = 2 * x_zrow();
+        e.field_stride = 2 * modes;
         return e;
     }
 
@@ -1438,7 +1443,8 @@ private:
     // directly) and the faster is kept.
     void make_fused() {
         for (std::size_t q = 0; q < groups_.size(); ++q)
-            if (ens_size(groups_[q]->n) && groups_[q]->line_layout() && static_cast<int>(fused_.size()) < kEnsGroups)
+            if (ens_size(groups_[q]->n) && !groups_[q]->transposed_layout() &&
+                static_cast<int>(fused_.size()) < kEnsGroups)
                 fused_.push_back(q);
         if (fused_.empty()) return;
         std::sort(fused_.begin(), fused_.end(), [&](std::size_t a, std::size_t b) { return groups_[a]->n > groups_[b]->n; });

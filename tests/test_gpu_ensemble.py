@@ -155,15 +155,19 @@ def test_config_64_members_100_steps():
         assert de <= 1e-4 * o.diagnostics(0.0)["dissipation"], j
 
 
-def test_one_launch_per_pass_matches_oracle(monkeypatch):
+@pytest.mark.parametrize("layout", ["0", "2"])
+def test_one_launch_per_pass_matches_oracle(monkeypatch, layout):
     """Every fused size (16, 32, 48, 64, 96, 128) in one launch per pass,
     plus a size outside that set (24) on its own chain beside them; FP64
-    members against their oracle solves."""
+    members against their oracle solves. Layout 2 puts the groups of 64, 96
+    and 128 in the ky-blocked layout (the y tiles then move pairs between it
+    and the x pass's fields)."""
     monkeypatch.setenv("SPH_ENSEMBLE_FUSED", "1")
+    monkeypatch.setenv("SPH_LAYOUT", layout)
     sizes = [128, 16, 24, 32, 48, 64, 96, 32]
     nus = [0.01, 0.02, 0.01, 0.005, 0.01, 0.02, 0.01, 0.03]
     seeds = [2000 + j for j in range(len(sizes))]
-    print("case ensemble one launch per pass fp64", sizes, flush=True)
+    print("case ensemble one launch per pass fp64 layout", layout, sizes, flush=True)
     ens = sph.Ensemble(sizes, nus, seeds, precision="fp64")
     assert ens.launch_mode()[0]
     orcs = [member_oracle(n, nu, s) for n, nu, s in zip(sizes, nus, seeds)]
@@ -178,11 +182,13 @@ def test_one_launch_per_pass_matches_oracle(monkeypatch):
         assert err < 1e-12, (j, err)
 
 
+@pytest.mark.parametrize("layout", ["0", "2"])
 @pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("fp64", 1e-14)])
-def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision, tol):
+def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision, tol, layout):
+    monkeypatch.setenv("SPH_LAYOUT", layout)
     sizes = [32, 48, 64, 96] * 2
     seeds = [3000 + j for j in range(len(sizes))]
-    print("case ensemble launch structures", precision, flush=True)
+    print("case ensemble launch structures", precision, "layout", layout, flush=True)
     states = []
     for mode in ("1", "0"):
         monkeypatch.setenv("SPH_ENSEMBLE_FUSED", mode)
