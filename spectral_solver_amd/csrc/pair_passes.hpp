@@ -143,10 +143,17 @@ __device__ __forceinline__ cx<T> shfl_xor1(cx<T> v) {
 
 // Cache policy of the z passes' pair-field rows (128-byte segments of one
 // tile): FP32 stores non-temporal (512^3: forward z + update + next inverse
-// z 3.41 -> 3.33 ms); loads default (NT loads: 3.48 ms).
+// z 3.41 -> 3.33 ms).
 template <typename T>
 constexpr int z_pair_store_policy() {
     return sizeof(T) == 4 ? kPolicyStream : kPolicyDefault;
+}
+// and the forward-z pass's loads of them: NT at N = 1024 in FP32 (27.9 ->
+// 27.2 ms per launch in the ky-blocked layout; neutral at 512); its state
+// accesses keep the default (NT state: 512^3 3.1 -> 4.1 ms)
+template <typename T, int N>
+constexpr int z_pair_load_policy() {
+    return sizeof(T) == 4 && N >= 1024 ? kPolicyStream : kPolicyDefault;
 }
 
 // Ping-pong z tiles (zfft's PP) in the solver's z passes: large lengths with
@@ -1066,7 +1073,7 @@ __device__ __forceinline__ void pfwd_z_rk_tile(const PairFwdZArgs<T>& a, TileId 
     for (int h = 0; h < HS; ++h) r0[h] = r1[h] = r2[h] = czero<T>();
     cx<T> v[1][T0][R0], w[1][TLS][RL], vn[T0][R0];
     auto load_pair = [&](int p, cx<T> (&dst)[T0][R0]) {
-        const Buf<T> b(pick(a.in, p) + zw + plane * 2, p_bytes);
+        const Buf<T, z_pair_load_policy<T, N>()> b(pick(a.in, p) + zw + plane * 2, p_bytes);
 #pragma unroll
         for (int i = 0; i < T0; ++i)
 #pragma unroll
