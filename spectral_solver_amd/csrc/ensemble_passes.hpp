@@ -106,7 +106,9 @@ __device__ __forceinline__ void ens_tile(const EnsGroupArgs<T>& a, TileId t, int
     } else if constexpr (PASS == 1 || PASS == 3) {
         constexpr bool INV = PASS == 1;
         if (a.blocked) {
-            if constexpr (N % (YL / 2) == 0 && N % zblock_ky<T, N>() == 0) {
+            // the ky-blocked layout exists from n = 64 on (pass_table.hpp)
+            if constexpr (N >= 64) {
+                static_assert(N % (YL / 2) == 0 && N % zblock_ky<T, N>() == 0, "blocked y tiles divide the lines");
                 // tile t.x of the group's pair lines, field by field; t.y the member
                 const long long per_field = static_cast<long long>(N / 2 + 1) * N / (YL / 2);
                 const int field = static_cast<int>(t.x / per_field);

@@ -238,7 +238,8 @@ struct Launch {
         }
     }
     // the ky-blocked spectral layout, at the lengths whose ky blocks and y
-    // tiles divide them
+    // tiles divide them (the one-launch ensemble kernels assume the same
+    // bound)
     static constexpr bool kZB = N >= 64;
     template <int L>
     static void plines_b_l(const cx<T>* in, cx<T>* out, int nfields, long long field_stride, long long member_stride,
