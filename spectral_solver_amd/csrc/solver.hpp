@@ -849,12 +849,14 @@ private:
             }
             return total;
         };
-        // the line layout is timed unless another one is forced (its
-        // forward-z pass also serves the transposed layout)
+        // the line layout's forward-z pass also serves the transposed layout
         int keep = 0;
         double best_other = 0.0;
         use_layout(0);
-        if (forced_layout != 1 && forced_layout != 2) {
+        // the line layout in full unless another one is forced and available
+        // (a forced layout this length or device lacks falls back to it)
+        const bool forced_other = (forced_layout == 1 && xt_allowed) || (forced_layout == 2 && zb_allowed);
+        if (!forced_other) {
             layout_ms_[0] = tune_layout(var_, tune_ms_, pt_->variants, false);
             best_other = layout_ms_[0];
         } else {
