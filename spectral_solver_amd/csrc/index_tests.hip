@@ -153,9 +153,76 @@ void register_dft() {
     }
 }
 
+// The ky-blocked spectral layout: spectral_mode_index is a bijection of
+// the n/2+1 kx planes of an n^3 grid for every ky block kb that divides n (kb
+// = n: the plain (kx*n + kz)*n + ky), spectral_mode_of inverts it, and the z
+// tiles' ZRow addresses the same modes.
+void blocked_modes(int n, int kb) {
+    const int H = n / 2 + 1;
+    const long long total = static_cast<long long>(H) * n * n;
+    std::vector<int> seen(static_cast<std::size_t>(total), 0);
+    for (int kx = 0; kx < H; ++kx)
+        for (int kz = 0; kz < n; ++kz)
+            for (int ky = 0; ky < n; ++ky) {
+                const long long i = sph::spectral_mode_index(kx, kz, ky, n, kb);
+                expect(i >= 0 && i < total, "blocked mode range", n, kb);
+                if (i < 0 || i >= total) continue;
+                ++seen[static_cast<std::size_t>(i)];
+                int x, z, y;
+                sph::spectral_mode_of(i, n, kb, x, z, y);
+                expect(x == kx && z == kz && y == ky, "blocked mode inverse", n, kb);
+                const sph::ZRow zr(ky, kb, n);
+                expect(static_cast<long long>(kx) * n * n + zr.base + static_cast<long long>(kz) * zr.step == i,
+                       "z-tile row of the blocked layout", n, kb);
+                if (kb == n)
+                    expect(i == (static_cast<long long>(kx) * n + kz) * n + ky, "plain layout", n, kz);
+            }
+    for (long long i = 0; i < total; ++i) expect(seen[static_cast<std::size_t>(i)] == 1, "blocked mode bijection", n, kb);
+}
+
+// k_plines_b's blocked side: the q-th value of a tile of NL lines (NL/2 pair
+// lines of consecutive kz) is element ky of line `line`, each exactly once,
+// at the offset of (ky, pair line line/2, member line&1) in the layout
+// relative to the tile's first value; consecutive q are consecutive offsets
+// within runs of NL*KB values (coalesced).
+template <int N, int NL, int KB>
+void plines_b_map(int z0) {
+    std::vector<int> seen(static_cast<std::size_t>(N) * NL, 0);
+    const long long base = sph::spectral_mode_index(0, z0, 0, N, KB) * 2;
+    long long prev = -1;
+    for (int q = 0; q < N * NL; ++q) {
+        int ky = -1, line = -1;
+        const long long o = sph::plines_b_slot<N, NL, KB>(q, ky, line);
+        expect(ky >= 0 && ky < N && line >= 0 && line < NL, "plines_b slot range", q, N);
+        if (ky < 0 || ky >= N || line < 0 || line >= NL) continue;
+        ++seen[static_cast<std::size_t>(ky) * NL + line];
+        const long long want = sph::spectral_mode_index(0, z0 + line / 2, ky, N, KB) * 2 + (line & 1) - base;
+        expect(o == want, "plines_b slot offset", q, N);
+        if (q % (NL * KB) != 0) expect(o == prev + 1, "plines_b run contiguous", q, N);
+        prev = o;
+    }
+    for (int a = 0; a < N * NL; ++a) expect(seen[static_cast<std::size_t>(a)] == 1, "plines_b slot bijection", N, a);
+}
+
 }  // namespace
 
 int main() {
+    for (int n : {64, 96, 128, 256}) {
+        blocked_modes(n, n);
+        for (int kb : {1, 2, 4, 8, 16})
+            if (n % kb == 0) blocked_modes(n, kb);
+    }
+    // the blocked y tiles the library instantiates: KB = the default z
+    // tile's NL/2, NL = 16, 8, 4, and a tile in the middle of the kz range
+    plines_b_map<512, 16, sph::zblock_ky<float, 512>()>(8);
+    plines_b_map<512, 8, sph::zblock_ky<float, 512>()>(500);
+    plines_b_map<512, 4, sph::zblock_ky<float, 512>()>(2);
+    plines_b_map<1024, 8, sph::zblock_ky<float, 1024>()>(1016);
+    plines_b_map<1024, 4, sph::zblock_ky<float, 1024>()>(4);
+    plines_b_map<96, 4, sph::zblock_ky<float, 96>()>(90);
+    plines_b_map<96, 16, sph::zblock_ky<float, 96>()>(8);
+    plines_b_map<512, 8, sph::zblock_ky<double, 512>()>(16);
+    plines_b_map<64, 32, sph::zblock_ky<float, 64>()>(16);
     for (int n : {8, 12, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024}) {
         swizzled_tile<2>(n);
         swizzled_tile<4>(n);

@@ -83,9 +83,23 @@ struct PairFwdZArgs {
     int kb;  // ky block of the state and of the pair fields (ny: plain layout)
 };
 
+// Mode offset of (kx, kz, ky) in one member's state or pair field of an n^3
+// grid (n*n modes per kx plane) with ky blocks of kb (kb = n: the plain
+// layout, (kx*n + kz)*n + ky), and its inverse.
+__host__ __device__ __forceinline__ long long spectral_mode_index(int kx, int kz, int ky, int n, int kb) {
+    return static_cast<long long>(kx) * n * n + static_cast<long long>(ky / kb) * n * kb +
+           static_cast<long long>(kz) * kb + ky % kb;
+}
+__host__ __device__ __forceinline__ void spectral_mode_of(long long i, int n, int kb, int& kx, int& kz, int& ky) {
+    kx = static_cast<int>(i / (static_cast<long long>(n) * n));
+    const int rem = static_cast<int>(i % (static_cast<long long>(n) * n));
+    kz = (rem % (n * kb)) / kb;
+    ky = (rem / (n * kb)) * kb + rem % kb;
+}
+
 // Mode offset within a kx plane of element (kz) 0 of line ky, and the
 // offset step between consecutive kz, for ky blocks of kb (kb = ny: the plain
-// layout, kz * ny + ky).
+// layout, kz * ny + ky); spectral_mode_index of (0, kz, ky) = base + kz*step.
 struct ZRow {
     uint32_t base, step;
     __host__ __device__ __forceinline__ ZRow(uint32_t ky, uint32_t kb, uint32_t n)
@@ -466,6 +480,19 @@ template <typename T, int N>
 constexpr int zblock_ky() {
     return ZTile<T, N>::NL / 2;
 }
+// Blocked side of a k_plines_b tile of NL lines: the q-th value in address
+// order is element ky of tile line `line` (pair line line/2, member line&1),
+// at this offset (complex values) from the tile's first blocked value (pair
+// line 0, ky = 0, member 0).
+template <int N, int NL, int KB>
+__host__ __device__ __forceinline__ long long plines_b_slot(int q, int& ky, int& line) {
+    constexpr int RUN = NL * KB;  // contiguous values per ky block
+    const int b = q / RUN, r = q % RUN;
+    const int g = r / (2 * KB), kyl = (r >> 1) % KB, f = r & 1;
+    ky = b * KB + kyl;
+    line = 2 * g + f;
+    return static_cast<long long>(b) * N * KB * 2 + r;
+}
 // One tile: pair lines z0 .. z0 + NL/2 - 1 of kx, the field and member
 // offsets already applied to src and dst (twl holds the twiddles, buf
 // tile_lds values).
@@ -474,20 +501,13 @@ __device__ __forceinline__ void plines_b_tile(const cx<T>* src, cx<T>* dst, long
                                               int z0, cx<T>* buf, const cx<T>* twl, int tid) {
     using TL = Tile<T, N, LINES>;
     constexpr int NL = TL::NL, NT = TL::NT, E = TL::E, KB = zblock_ky<T, N>();
-    constexpr int RUN = NL * KB;  // contiguous values per ky block on the blocked side
     static_assert(NL % 2 == 0, "pair tiles need an even line count");
     static_assert(N % (NL / 2) == 0 && N % KB == 0, "tiles must not straddle a kx plane or a ky block");
     constexpr bool SWZ = plines_swizzled<T, N>(NL);
     const long long line0 = kx * xs_kx + z0 * xs_z;  // line side: the tile's first value
     const long long blk0 = (static_cast<long long>(kx) * N * N + static_cast<long long>(z0) * KB) * 2;
     // blocked side: q-th value of the tile in address order -> offset, ky, line
-    auto blk = [&](int q, int& ky, int& line) -> long long {
-        const int b = q / RUN, r = q % RUN;
-        const int g = r / (2 * KB), kyl = (r >> 1) % KB, f = r & 1;
-        ky = b * KB + kyl;
-        line = 2 * g + f;
-        return blk0 + static_cast<long long>(b) * N * KB * 2 + r;
-    };
+    auto blk = [&](int q, int& ky, int& line) -> long long { return blk0 + plines_b_slot<N, NL, KB>(q, ky, line); };
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         const int q = tid + i * NT;
@@ -1210,10 +1230,8 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_div3(const cx<T>* u3,
     double m = 0.0;
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
          i += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const int kx = static_cast<int>(i / (static_cast<long long>(n) * n));
-        const int rem = static_cast<int>(i % (static_cast<long long>(n) * n));
-        const int kz = (rem % (n * kb)) / kb;
-        const int ky = (rem / (n * kb)) * kb + rem % kb;
+        int kx, kz, ky;
+        spectral_mode_of(i, n, kb, kx, kz, ky);
         const T fx = T(kx), fy = T(signed_mode(ky, n)), fz = T(signed_mode(kz, n));
         const cx<T> d = (u3[3 * i] * fx + u3[3 * i + 1] * fy) + u3[3 * i + 2] * fz;
         m = fmax(m, static_cast<double>(sqrt(d.x * d.x + d.y * d.y)));

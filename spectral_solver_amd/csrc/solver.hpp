@@ -955,14 +955,13 @@ private:
         pairs_valid_ = false;
     }
     // f(plain mode index, device mode index) over all modes of one member
-    // (ky blocks of kb_: device index kx*n*n + (ky/kb)*n*kb + kz*kb + ky%kb)
+    // (ky blocks of kb_, spectral_mode_index)
     template <typename F>
     void for_each_mode_block(F&& f) const {
-        const std::int64_t nn = static_cast<std::int64_t>(n) * n;
-        for (std::int64_t kx = 0; kx < H; ++kx)
-            for (std::int64_t kz = 0; kz < n; ++kz)
-                for (std::int64_t ky = 0; ky < n; ++ky)
-                    f((kx * n + kz) * n + ky, kx * nn + (ky / kb_) * n * kb_ + kz * kb_ + ky % kb_);
+        for (int kx = 0; kx < H; ++kx)
+            for (int kz = 0; kz < n; ++kz)
+                for (int ky = 0; ky < n; ++ky)
+                    f(spectral_mode_index(kx, kz, ky, n, n), spectral_mode_index(kx, kz, ky, n, kb_));
     }
 
     const PassTable<T>* pt_ = nullptr;

@@ -1,6 +1,7 @@
 """Host checks of the pass kernels' index maps (LDS layouts, butterfly and
-slot orders, the LDS stash of the fused forward-z kernel): each must be a
-bijection and fit the kernel's LDS. The checks are compiled from the same
+slot orders, the LDS stash of the fused forward-z kernel, the ky-blocked
+spectral layout's mode map and the blocked y tiles' value order): each must
+be a bijection and fit the kernel's LDS. The checks are compiled from the same
 headers as the kernels (spectral_solver_amd/csrc/index_tests.hip) and run on
 the host."""
 import os
