@@ -155,6 +155,14 @@ int sph_solver_set_uhat(sph_solver_t solver, int component, const double* host);
 /* Counters and timers behind print_profile (impl.hpp:1181-1286). */
 int sph_solver_profile(sph_solver_t solver, sph_profile* out);
 
+/* GPU time in ms of each of the most recent RK3 steps (HIP events recorded
+ * on the solver's stream before and after each step, graph replay
+ * included), oldest first: up to `max` values into ms, their number in
+ * *count (at most the last 4096 steps are kept). No reference counterpart
+ * (the reference accumulates wall time only, impl.hpp:980,1043); bench.py
+ * reports the median. */
+int sph_solver_step_times(sph_solver_t solver, double* ms, int max, int* count);
+
 /* print_profile(rank, init, run) (impl.hpp:1181-1286): prints the
  * profile_solver / profile_breakdown / profile_fft lines to stdout. */
 int sph_solver_print_profile(sph_solver_t solver, int rank, double init_time, double run_time);

@@ -83,6 +83,7 @@ SYMBOLS = [
     ("sph_solver_get_uhat", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     ("sph_solver_set_uhat", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     ("sph_solver_profile", ctypes.c_int, [_VP, ctypes.POINTER(_Profile)]),
+    ("sph_solver_step_times", ctypes.c_int, [_VP, _DP, ctypes.c_int, _IP]),
     ("sph_solver_print_profile", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     ("sph_solver_set_graph", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sph_solver_synchronize", ctypes.c_int, [_VP]),
@@ -373,6 +374,14 @@ class TaylorGreenSolver:
         p = _Profile()
         _check(load_library().sph_solver_profile(self._h, ctypes.byref(p)))
         return {f[0]: getattr(p, f[0]) for f in _Profile._fields_}
+
+    def step_times(self, last: int = 4096) -> np.ndarray:
+        """GPU time (ms, HIP events around each step) of up to `last` of
+        the most recent RK3 steps, oldest first."""
+        ms = np.zeros(max(int(last), 0))
+        n = ctypes.c_int()
+        _check(load_library().sph_solver_step_times(self._h, _dptr(ms), len(ms), ctypes.byref(n)))
+        return ms[:n.value].copy()
 
     def time_passes(self, reps: int = 5):
         ms = np.zeros(len(_PASS_NAMES))

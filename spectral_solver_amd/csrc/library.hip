@@ -374,6 +374,15 @@ int sph_solver_profile(sph_solver_t solver, sph_profile* out) {
     });
 }
 
+int sph_solver_step_times(sph_solver_t solver, double* ms, int max, int* count) {
+    return guarded([&] {
+        require_handle(solver);
+        require_pointer(count);
+        if (max > 0) require_pointer(ms);
+        *count = solver->impl->step_times(ms, max);
+    });
+}
+
 int sph_solver_print_profile(sph_solver_t solver, int rank, double init_time, double run_time) {
     return guarded([&] {
         require_handle(solver);

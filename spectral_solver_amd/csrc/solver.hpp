@@ -1014,6 +1014,12 @@ public:
         while (!pending_.empty()) resolve_one();
         return total_;
     }
+    // GPU time of each of the last kKeep steps (ms), oldest first
+    static constexpr std::size_t kKeep = 4096;
+    const std::deque<float>& step_ms() {
+        while (!pending_.empty()) resolve_one();
+        return recent_;
+    }
     long long steps = 0;
 
 private:
@@ -1024,10 +1030,13 @@ private:
         float e = 0.f;
         hip_check(hipEventElapsedTime(&e, p.first, p.second), "hipEventElapsedTime");
         total_ += 1e-3 * e;
+        recent_.push_back(e);
+        if (recent_.size() > kKeep) recent_.pop_front();
         (void)hipEventDestroy(p.first);
         (void)hipEventDestroy(p.second);
     }
     std::deque<std::pair<hipEvent_t, hipEvent_t>> pending_;
+    std::deque<float> recent_;
     double total_ = 0.0;
 };
 
@@ -1072,6 +1081,9 @@ public:
     virtual void get_uhat(int c, double* host) = 0;
     virtual void set_uhat(int c, const double* host) = 0;
     virtual void profile(sph_profile* out) = 0;
+    // GPU time (ms, HIP events around each step) of up to `max` of the most
+    // recent steps, oldest first; returns how many were written
+    virtual int step_times(double* ms, int max) = 0;
     virtual void set_graph(bool on) = 0;
     virtual void synchronize() = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
@@ -1192,6 +1204,12 @@ public:
     void profile(sph_profile* out) override {
         prof_.step = timer_.seconds();
         *out = prof_;
+    }
+    int step_times(double* ms, int max) override {
+        const std::deque<float>& r = timer_.step_ms();
+        const int n = static_cast<int>(std::min<std::size_t>(r.size(), max > 0 ? static_cast<std::size_t>(max) : 0));
+        for (int i = 0; i < n; ++i) ms[i] = r[r.size() - n + i];
+        return n;
     }
     void set_graph(bool on) override { use_graph_ = on; }
     void synchronize() override { hip_check(hipStreamSynchronize(stream_.get()), "hipStreamSynchronize"); }

@@ -221,6 +221,20 @@ def test_graph_and_direct_launch_agree_bitwise():
         assert np.array_equal(a.u_hat(c), b.u_hat(c))
 
 
+def test_step_times_are_per_step_gpu_times():
+    """sph_solver_step_times: one positive GPU time per step, oldest first,
+    at most the number asked for, summing to the profile's step time."""
+    s = sph.TaylorGreenSolver(64, 1600.0, "fp32")
+    for _ in range(5):
+        s.step_rk3(1e-3)
+    t = s.step_times()
+    assert len(t) == 5 and np.all(t > 0)
+    assert np.array_equal(s.step_times(3), t[2:])
+    assert len(s.step_times(0)) == 0
+    assert abs(t.sum() * 1e-3 - s.profile()["step"]) <= 1e-9 + 1e-6 * s.profile()["step"]
+    s.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_calls_between_steps_leave_the_trajectory_unchanged(precision):
     """Each stage's last kernel also computes the next stage's inverse z
