@@ -668,10 +668,11 @@ private:
     // precision, n, members, forced layout) before timing, and a new one is
     // appended after it. One line per choice:
     //   sph-tune-2 <build> <device> <bytes> <n> <members> <forced layout> <layout>
+    // (forced layout: SPH_LAYOUT or -1, plus 10 with SPH_X_ZMAJOR=0)
     //     <var x4> <xvar x4> <bvar x4> <layout ms x4> <ms x16> <xms x16> <bms x16>
     // (the last matching line wins). Entries whose variants do not exist in
     // this build or whose layout does not fit are ignored.
-    static std::string tune_key(int forced_layout, int n_, int members) {
+    static std::string tune_key(int layout_key, int n_, int members) {
         int dev = 0;
         hipDeviceProp_t prop{};
         std::string device = "unknown";
@@ -680,7 +681,7 @@ private:
         for (char& ch : device)
             if (ch == ' ') ch = '_';
         return std::string("sph-tune-2 ") + SPH_BUILD_ID + " " + device + " " + std::to_string(sizeof(T)) + " " +
-               std::to_string(n_) + " " + std::to_string(members) + " " + std::to_string(forced_layout);
+               std::to_string(n_) + " " + std::to_string(members) + " " + std::to_string(layout_key);
     }
     bool choice_valid(const Choice& c, bool xt_allowed, bool zb_allowed) const {
         if (c.layout < 0 || c.layout > 2) return false;
@@ -789,7 +790,10 @@ private:
         }
         static std::mutex mu;
         static std::map<std::tuple<int, int, int, int>, Choice> cache;
-        const auto key = std::make_tuple(static_cast<int>(sizeof(T)), n, count, forced_layout);
+        // the forced layout and SPH_X_ZMAJOR=0 (+10) key the choice
+        const char* xz = std::getenv("SPH_X_ZMAJOR");
+        const int layout_key = forced_layout + (xz && xz[0] == '0' ? 10 : 0);
+        const auto key = std::make_tuple(static_cast<int>(sizeof(T)), n, count, layout_key);
         {
             std::lock_guard<std::mutex> lock(mu);
             const auto it = cache.find(key);
@@ -804,7 +808,7 @@ private:
             return;
         }
         const char* tune_file = std::getenv("SPH_TUNE_CACHE");
-        const std::string file_key = tune_file && tune_file[0] ? tune_key(forced_layout, n, count) : std::string();
+        const std::string file_key = tune_file && tune_file[0] ? tune_key(layout_key, n, count) : std::string();
         if (!file_key.empty()) {
             Choice c{};
             if (load_tune_file(tune_file, file_key, c) && choice_valid(c, xt_allowed, zb_allowed)) {
