@@ -106,8 +106,9 @@ def ensemble_leg(sph, steps: int, warmup: int):
     ens.close()
     return {"workload": "64 solves: 16 each of 32^3, 48^3, 64^3, 96^3, FP32, nu=1e-2, CFL 0.5 dt, "
                         f"{steps} RK3 steps, one hipGraph per step",
-            "one_launch_per_pass": mode[0],
-            "timed_step_ms": {"per_size_chains": round(mode[1][0], 4), "one_launch_per_pass": round(mode[1][1], 4)},
+            "launch_structure": ("per_size_chains", "one_launch_per_pass", "one_launch_per_stage")[mode[0]],
+            "timed_step_ms": {"per_size_chains": round(mode[1][0], 4), "one_launch_per_pass": round(mode[1][1], 4),
+                              "one_launch_per_stage": round(mode[1][2], 4)},
             "mpts_per_s": round(pts / sec / 1e6, 3), "ms_per_step": round(sec * 1e3, 4),
             "points": pts, "algorithmic_bytes_per_step": alg,
             "roofline_frac": round(alg / sec / 1e9 / HBM_PEAK_GBS, 4),

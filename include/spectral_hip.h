@@ -322,15 +322,18 @@ int sph_ensemble_step_seconds(sph_ensemble_t ens, double* seconds, int64_t* step
 /* Pass timings as sph_solver_time_passes, summed over the distinct sizes. */
 int sph_ensemble_time_passes(sph_ensemble_t ens, int reps, double* ms, double* bytes);
 
-/* How the ensemble launches a step. Members of the sizes 16, 32, 48, 64, 96
- * and 128 can run as one launch per pass for all of them (four launches per
- * RK stage, one-wavefront tiles of every size in each launch); other sizes,
- * and all sizes when that is slower, run one chain of launches per size on
- * parallel branches of the step's graph. Chosen at creation by timing both
- * (SPH_ENSEMBLE_FUSED=0 / 1 forces it). *one_launch_per_pass = 1 when the
- * former is used; step_ms[0], step_ms[1] = the timed step with per-size
- * chains and with one launch per pass (0 where not timed). */
-int sph_ensemble_launch_mode(sph_ensemble_t ens, int* one_launch_per_pass, double* step_ms);
+/* How the ensemble launches a step (*mode):
+ *   0  one chain of launches per size (four per RK stage) on parallel
+ *      branches of the step's graph;
+ *   1  one launch per pass for all members of the sizes 16, 32, 48, 64, 96
+ *      and 128 (four launches per RK stage, one-wavefront tiles of every size
+ *      in each launch); other sizes on their own chains beside it;
+ *   2  one launch per RK stage for those sizes: a cooperative launch of
+ *      resident work-groups that walk the tiles of the four passes with a
+ *      grid-wide barrier between passes (not captured into the graph).
+ * Chosen at creation by timing each (SPH_ENSEMBLE_FUSED=0 / 1 / 2 forces
+ * one). step_ms (3 doubles) = the timed step of each (0 where not timed). */
+int sph_ensemble_launch_mode(sph_ensemble_t ens, int* mode, double* step_ms);
 
 /* The ensemble's initial field for one member, on the host: three
  * components of n*n*(n/2+1) interleaved (re, im) doubles, layout as

@@ -566,11 +566,12 @@ class Ensemble:
         return {k: (float(m), float(b)) for k, m, b in zip(_PASS_NAMES, ms, by)}
 
     def launch_mode(self):
-        """(one launch per pass for the fused sizes in use, [step ms with
-        per-size chains, with one launch per pass])."""
-        f, ms = ctypes.c_int(), np.zeros(2)
+        """(launch structure in use: 0 per-size chains, 1 one launch per
+        pass, 2 one launch per stage; [step ms with each, 0 where not
+        timed])."""
+        f, ms = ctypes.c_int(), np.zeros(3)
         _check(load_library().sph_ensemble_launch_mode(self._h, ctypes.byref(f), _dptr(ms)))
-        return bool(f.value), [float(x) for x in ms]
+        return int(f.value), [float(x) for x in ms]
 
 
 def run_taylor_green(config: RunConfig) -> Diagnostics:

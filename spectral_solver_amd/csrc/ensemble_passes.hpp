@@ -15,6 +15,8 @@
 // group boundaries and grids are kernel arguments.
 #pragma once
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "pair_passes.hpp"
 
 namespace sph {
@@ -136,13 +138,11 @@ __device__ __forceinline__ void ens_tile(const EnsGroupArgs<T>& a, TileId t, int
 // 3 forward y, 4 forward z + update + the next stage's inverse z. Two waves
 // per SIMD at least (a 256-register budget: the 96^3 x tile needs 189, and
 // held to 128 the forward-z tiles spill).
+// Logical block b of a launch over `grid` -> (group q, its tile t), and that
+// tile's pass PASS.
 template <typename T, int PASS>
-__global__ void __launch_bounds__(kEnsThreads, 2)
-    k_ens_pass(const EnsGroupArgs<T>* __restrict__ table, EnsGrid grid, int stage) {
-    __shared__ cx<T> lds[ens_lds<T>()];
-    __shared__ cx<T> twl[kEnsMaxN];
-    const int tid = threadIdx.x;
-    const unsigned b = ens_block(grid.total);
+__device__ __forceinline__ void ens_run_block(const EnsGroupArgs<T>* __restrict__ table, const EnsGrid& grid,
+                                              unsigned b, int stage, cx<T>* lds, cx<T>* twl, int tid) {
     int q = 0;
     unsigned first = grid.first[0], gx = grid.gx[0], gy = grid.gy[0];
 #pragma unroll
@@ -166,6 +166,54 @@ __global__ void __launch_bounds__(kEnsThreads, 2)
         default:
             break;
     }
+}
+
+template <typename T, int PASS>
+__global__ void __launch_bounds__(kEnsThreads, 2)
+    k_ens_pass(const EnsGroupArgs<T>* __restrict__ table, EnsGrid grid, int stage) {
+    __shared__ cx<T> lds[ens_lds<T>()];
+    __shared__ cx<T> twl[kEnsMaxN];
+    ens_run_block<T, PASS>(table, grid, ens_block(grid.total), stage, lds, twl, threadIdx.x);
+}
+
+// One launch per RK stage: passes 1-4 of every fused group in one
+// cooperative launch of resident work-groups, each walking the tiles of a
+// pass and meeting the others at a grid-wide barrier before the next pass
+// (cooperative_groups' grid sync: the launch is refused, not left waiting,
+// when the grid does not fit the device at once). Work-group w runs on XCD
+// w % 8 (the dispatcher's round robin) and takes tiles of that XCD's
+// contiguous eighth of each pass, as ens_block arranges for the per-pass
+// launches. gridDim.x is a multiple of 8.
+struct EnsStageGrids {
+    EnsGrid pass[4];  // passes 1..4
+};
+template <typename T, int PASS>
+__device__ __forceinline__ void ens_stage_pass(const EnsGroupArgs<T>* __restrict__ table, const EnsGrid& grid,
+                                               int stage, cx<T>* lds, cx<T>* twl, int tid) {
+    constexpr unsigned kXcds = 8;
+    const unsigned per_xcd = (grid.total + kXcds - 1) / kXcds;
+    const unsigned xcd = blockIdx.x % kXcds, wi = blockIdx.x / kXcds, wn = gridDim.x / kXcds;
+    const unsigned lo = xcd * per_xcd;
+    const unsigned hi = lo + per_xcd < grid.total ? lo + per_xcd : grid.total;
+    for (unsigned b = lo + wi; b < hi; b += wn) {
+        ens_run_block<T, PASS>(table, grid, b, stage, lds, twl, tid);
+        __syncthreads();  // the next tile rewrites the LDS tiles and twiddles
+    }
+}
+template <typename T>
+__global__ void __launch_bounds__(kEnsThreads, 2)
+    k_ens_stage(const EnsGroupArgs<T>* __restrict__ table, EnsStageGrids grids, int stage) {
+    __shared__ cx<T> lds[ens_lds<T>()];
+    __shared__ cx<T> twl[kEnsMaxN];
+    const int tid = threadIdx.x;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    ens_stage_pass<T, 1>(table, grids.pass[0], stage, lds, twl, tid);
+    grid.sync();
+    ens_stage_pass<T, 2>(table, grids.pass[1], stage, lds, twl, tid);
+    grid.sync();
+    ens_stage_pass<T, 3>(table, grids.pass[2], stage, lds, twl, tid);
+    grid.sync();
+    ens_stage_pass<T, 4>(table, grids.pass[3], stage, lds, twl, tid);
 }
 
 // Host: the launch grid of pass `pass` for groups of sizes n[q] with
@@ -215,5 +263,13 @@ EnsGrid ens_grid(int pass, int groups, const int* n, const int* members) {
 // Launches pass `pass` (stage `stage` for pass 4) over the table's groups.
 template <typename T>
 void ens_launch(int pass, const EnsGroupArgs<T>* table, const EnsGrid& grid, int stage, hipStream_t s);
+
+// Resident work-groups of k_ens_stage on this device (a multiple of 8; 0 if
+// cooperative launches are not supported), and its launch for RK stage
+// `stage` over passes 1-4 with that many work-groups.
+template <typename T>
+int ens_stage_blocks();
+template <typename T>
+void ens_launch_stage(const EnsGroupArgs<T>* table, const EnsStageGrids& grids, int blocks, int stage, hipStream_t s);
 
 }  // namespace sph

@@ -725,12 +725,12 @@ int sph_ensemble_time_passes(sph_ensemble_t ens, int reps, double* ms, double* b
     });
 }
 
-int sph_ensemble_launch_mode(sph_ensemble_t ens, int* one_launch_per_pass, double* step_ms) {
+int sph_ensemble_launch_mode(sph_ensemble_t ens, int* mode, double* step_ms) {
     return guarded([&] {
         require_handle(ens);
-        require_pointer(one_launch_per_pass);
+        require_pointer(mode);
         require_pointer(step_ms);
-        ens->impl->launch_mode(one_launch_per_pass, step_ms);
+        ens->impl->launch_mode(mode, step_ms);
     });
 }
 

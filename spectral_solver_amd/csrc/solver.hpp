@@ -1321,7 +1321,7 @@ public:
     virtual void synchronize() = 0;
     virtual double step_seconds(long long* steps) = 0;
     virtual void time_passes(int reps, double* ms, double* bytes) = 0;
-    virtual void launch_mode(int* fused, double* ms) const = 0;
+    virtual void launch_mode(int* mode, double* ms) const = 0;
     std::vector<int> sizes;
 };
 
@@ -1373,7 +1373,8 @@ public:
         auto ev = timer_.begin(s);
         for (auto& g : groups_) g->prepare_pairs(s);
         auto record = [&] { record_step(s); };
-        if (use_graph_) {
+        // the cooperative per-stage launch is not captured into the graph
+        if (use_graph_ && mode_ != 2) {
             graph_.launch(s, record);
         } else {
             record();
@@ -1386,10 +1387,9 @@ public:
         const auto w = locate(member);
         return groups_[w.first]->diagnostics(w.second, t, stream_.get());
     }
-    void launch_mode(int* fused, double* ms) const override {
-        *fused = use_fused_ ? 1 : 0;
-        ms[0] = mode_ms_[0];
-        ms[1] = mode_ms_[1];
+    void launch_mode(int* mode, double* ms) const override {
+        *mode = mode_;
+        for (int i = 0; i < 3; ++i) ms[i] = mode_ms_[i];
     }
     double max_velocity(int member) override {
         const auto w = locate(member);
@@ -1436,8 +1436,8 @@ private:
     void record_step(hipStream_t s) {
         std::vector<std::size_t> own;
         for (std::size_t q = 0; q < groups_.size(); ++q)
-            if (!(use_fused_ && is_fused(q))) own.push_back(q);
-        const bool fused = use_fused_ && !fused_.empty();
+            if (!(mode_ > 0 && is_fused(q))) own.push_back(q);
+        const bool fused = mode_ > 0 && !fused_.empty();
         if (!fused && own.size() == 1) {
             for (int st = 0; st < 3; ++st) groups_[own[0]]->launch_stage(st, s);
             return;
@@ -1451,7 +1451,9 @@ private:
                 hip_check(hipEventRecord(join_[q], b), "hipEventRecord");
             }
         }
-        if (fused)
+        if (fused && mode_ == 2)
+            for (int st = 0; st < 3; ++st) ens_launch_stage<T>(table_.get(), stage_grids_, stage_blocks_, st, s);
+        else if (fused)
             for (int st = 0; st < 3; ++st)
                 for (int k = 1; k < Group<T>::kPasses; ++k) ens_launch<T>(k, table_.get(), grid_[k], st, s);
         for (std::size_t q : own) hip_check(hipStreamWaitEvent(s, join_[q], 0), "hipStreamWaitEvent");
@@ -1470,6 +1472,7 @@ private:
                 static_cast<int>(fused_.size()) < kEnsGroups)
                 fused_.push_back(q);
         if (fused_.empty()) return;
+        stage_blocks_ = ens_stage_blocks<T>();
         std::sort(fused_.begin(), fused_.end(), [&](std::size_t a, std::size_t b) { return groups_[a]->n > groups_[b]->n; });
         std::vector<EnsGroupArgs<T>> args;
         std::vector<int> fn, fm;
@@ -1483,17 +1486,22 @@ private:
                   "hipMemcpy ensemble table");
         for (int k = 0; k < Group<T>::kPasses; ++k)
             grid_[k] = ens_grid<T>(k, static_cast<int>(fused_.size()), fn.data(), fm.data());
+        for (int k = 1; k < Group<T>::kPasses; ++k) stage_grids_.pass[k - 1] = grid_[k];
+        // SPH_ENSEMBLE_FUSED=0 / 1 / 2 forces per-size chains / one launch
+        // per pass / one launch per stage (where cooperative launches exist)
         const char* env = std::getenv("SPH_ENSEMBLE_FUSED");
-        if (env && (env[0] == '0' || env[0] == '1') && env[1] == '\0') {
-            use_fused_ = env[0] == '1';
+        if (env && env[0] >= '0' && env[0] <= '2' && env[1] == '\0') {
+            mode_ = env[0] - '0';
+            if (mode_ == 2 && stage_blocks_ < 8) throw std::runtime_error("one launch per stage needs cooperative launches");
             return;
         }
         const hipStream_t s = stream_.get();
         hipEvent_t a, b;
         hip_check(hipEventCreate(&a), "hipEventCreate");
         hip_check(hipEventCreate(&b), "hipEventCreate");
-        for (int mode = 0; mode < 2; ++mode) {
-            use_fused_ = mode == 1;
+        for (int mode = 0; mode < 3; ++mode) {
+            if (mode == 2 && stage_blocks_ < 8) break;
+            mode_ = mode;
             record_step(s);
             hip_check(hipEventRecord(a, s), "hipEventRecord");
             for (int r = 0; r < 2; ++r) record_step(s);
@@ -1506,7 +1514,9 @@ private:
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         hip_check(hipGetLastError(), "ensemble launch timing");
-        use_fused_ = mode_ms_[1] < mode_ms_[0];
+        mode_ = 0;
+        for (int mode = 1; mode < 3; ++mode)
+            if (mode_ms_[mode] > 0.0 && mode_ms_[mode] < mode_ms_[mode_]) mode_ = mode;
     }
 
     void make_branches() {
@@ -1533,12 +1543,15 @@ private:
     // one launch per pass: the fused groups (indices into groups_), their
     // argument table on the device, the grid of each pass, whether in use,
     // and the timed step of each launch structure (ms: per-group chains,
-    // one launch per pass)
+    // one launch per pass, one launch per stage); mode_ the one in use (0,
+    // 1, 2), stage_blocks_ the resident grid of the per-stage launch
     std::vector<std::size_t> fused_;
     DeviceArray<EnsGroupArgs<T>> table_;
     EnsGrid grid_[Group<T>::kPasses] = {};
-    bool use_fused_ = false;
-    double mode_ms_[2] = {0.0, 0.0};
+    EnsStageGrids stage_grids_{};
+    int stage_blocks_ = 0;
+    int mode_ = 0;
+    double mode_ms_[3] = {0.0, 0.0, 0.0};
 
 public:
     ~Ensemble() override {

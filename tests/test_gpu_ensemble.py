@@ -71,12 +71,13 @@ def test_fp64_members_match_their_oracle_solves():
         assert abs(ens.max_velocity(j) - o.max_velocity()) <= 1e-12 * o.max_velocity()
 
 
-@pytest.mark.parametrize("fused", ["0", "1"])
+@pytest.mark.parametrize("fused", ["0", "1", "2"])
 def test_graph_and_direct_launches_agree_bitwise(monkeypatch, fused):
-    # the launch structure (per-size chains or one launch per pass, whose
-    # tiles round differently in FP32) is forced: left to the timing at
-    # creation, two ensembles may choose differently
-    print("case ensemble graph, one launch per pass:", fused, flush=True)
+    # the launch structure (per-size chains, one launch per pass or per
+    # stage, whose tiles round differently in FP32) is forced: left to the
+    # timing at creation, two ensembles may choose differently; one launch
+    # per stage is never captured, so there both run directly
+    print("case ensemble graph, launch structure:", fused, flush=True)
     monkeypatch.setenv("SPH_ENSEMBLE_FUSED", fused)
     sizes, nus, seeds = [32, 48, 32], [0.01, 0.01, 0.02], [1, 2, 3]
     a = sph.Ensemble(sizes, nus, seeds, precision="fp32")
@@ -85,7 +86,7 @@ def test_graph_and_direct_launches_agree_bitwise(monkeypatch, fused):
     for _ in range(3):
         a.step_rk3([4e-3, 3e-3, 4e-3])
         b.step_rk3([4e-3, 3e-3, 4e-3])
-    assert a.launch_mode()[0] == b.launch_mode()[0] == (fused == "1")
+    assert a.launch_mode()[0] == b.launch_mode()[0] == int(fused)
     for j in range(3):
         for c in range(3):
             assert np.array_equal(a.u_hat(j, c), b.u_hat(j, c))
@@ -155,21 +156,22 @@ def test_config_64_members_100_steps():
         assert de <= 1e-4 * o.diagnostics(0.0)["dissipation"], j
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
 @pytest.mark.parametrize("layout", ["0", "2"])
-def test_one_launch_per_pass_matches_oracle(monkeypatch, layout):
-    """Every fused size (16, 32, 48, 64, 96, 128) in one launch per pass,
-    plus a size outside that set (24) on its own chain beside them; FP64
-    members against their oracle solves. Layout 2 puts the groups of 64, 96
-    and 128 in the ky-blocked layout (the y tiles then move pairs between it
-    and the x pass's fields)."""
-    monkeypatch.setenv("SPH_ENSEMBLE_FUSED", "1")
+def test_one_launch_per_pass_matches_oracle(monkeypatch, layout, mode):
+    """Every fused size (16, 32, 48, 64, 96, 128) in one launch per pass
+    (mode 1) or per stage (mode 2), plus a size outside that set (24) on its
+    own chain beside them; FP64 members against their oracle solves. Layout
+    2 puts the groups of 64, 96 and 128 in the ky-blocked layout (the y
+    tiles then move pairs between it and the x pass's fields)."""
+    monkeypatch.setenv("SPH_ENSEMBLE_FUSED", mode)
     monkeypatch.setenv("SPH_LAYOUT", layout)
     sizes = [128, 16, 24, 32, 48, 64, 96, 32]
     nus = [0.01, 0.02, 0.01, 0.005, 0.01, 0.02, 0.01, 0.03]
     seeds = [2000 + j for j in range(len(sizes))]
-    print("case ensemble one launch per pass fp64 layout", layout, sizes, flush=True)
+    print("case ensemble launch structure", mode, "fp64 layout", layout, sizes, flush=True)
     ens = sph.Ensemble(sizes, nus, seeds, precision="fp64")
-    assert ens.launch_mode()[0]
+    assert ens.launch_mode()[0] == int(mode)
     orcs = [member_oracle(n, nu, s) for n, nu, s in zip(sizes, nus, seeds)]
     dt = [1e-3 * (1 + 0.1 * j) for j in range(len(sizes))]
     for _ in range(3):
@@ -182,18 +184,19 @@ def test_one_launch_per_pass_matches_oracle(monkeypatch, layout):
         assert err < 1e-12, (j, err)
 
 
+@pytest.mark.parametrize("fused", ["1", "2"])
 @pytest.mark.parametrize("layout", ["0", "2"])
 @pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("fp64", 1e-14)])
-def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision, tol, layout):
+def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision, tol, layout, fused):
     monkeypatch.setenv("SPH_LAYOUT", layout)
     sizes = [32, 48, 64, 96] * 2
     seeds = [3000 + j for j in range(len(sizes))]
-    print("case ensemble launch structures", precision, "layout", layout, flush=True)
+    print("case ensemble launch structures", fused, "vs 0", precision, "layout", layout, flush=True)
     states = []
-    for mode in ("1", "0"):
+    for mode in (fused, "0"):
         monkeypatch.setenv("SPH_ENSEMBLE_FUSED", mode)
         ens = sph.Ensemble(sizes, 1e-2, seeds, precision=precision)
-        assert ens.launch_mode()[0] == (mode == "1")
+        assert ens.launch_mode()[0] == int(mode)
         dt = cfl_dt(ens)
         for _ in range(4):
             ens.step_rk3(dt)
@@ -205,8 +208,9 @@ def test_one_launch_per_pass_agrees_with_per_size_chains(monkeypatch, precision,
         assert np.sqrt(num / den) <= tol, (j, np.sqrt(num / den))
 
 
-def test_launch_mode_reports_both_timings():
+def test_launch_mode_reports_every_timing():
     ens = sph.Ensemble([32, 48, 64, 96], 1e-2, [1, 2, 3, 4], precision="fp32")
-    fused, ms = ens.launch_mode()
-    print("  one launch per pass:", fused, "step ms per-size chains / one launch per pass:", ms, flush=True)
-    assert ms[0] > 0 and ms[1] > 0 and fused == (ms[1] < ms[0])
+    mode, ms = ens.launch_mode()
+    print("  launch structure:", mode, "step ms per-size chains / one launch per pass / per stage:", ms, flush=True)
+    assert all(x > 0 for x in ms)
+    assert mode == int(np.argmin(ms))
