@@ -200,6 +200,10 @@ __device__ __forceinline__ void ens_stage_pass(const EnsGroupArgs<T>* __restrict
         __syncthreads();  // the next tile rewrites the LDS tiles and twiddles
     }
 }
+// (two waves per SIMD like the per-pass kernels: the four passes' tile code
+// in one kernel then spills 476 bytes per lane; held to one wave per SIMD it
+// fits in 256 + 138 registers and runs slower, 12.7 against 12.1 ms per
+// 64-member step)
 template <typename T>
 __global__ void __launch_bounds__(kEnsThreads, 2)
     k_ens_stage(const EnsGroupArgs<T>* __restrict__ table, EnsStageGrids grids, int stage) {
