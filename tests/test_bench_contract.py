@@ -47,3 +47,8 @@ def test_json_line_small_grid():
     assert d["cpu_baseline"] is None  # --no-cpu
     # the FP64 leg ran and its error against FP32 is small at this size
     assert d["fp64"]["ms_per_step"] > 0 and d["dissipation_rel_err_fp32_vs_fp64"] < 1e-4
+    # the GPU times of the timed steps (events around each graph replay) sit
+    # below the host clock over them, and the layout report names a layout
+    assert 0 < d["ms_per_step_gpu_median"] <= 1.5 * d["ms_per_step"]
+    assert 0 < d["fp64"]["ms_per_step_gpu_median"] <= 1.5 * d["fp64"]["ms_per_step"]
+    assert rf["work_layout"]["layout"] in ("line", "transposed", "ky-blocked")
