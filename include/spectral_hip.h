@@ -296,7 +296,7 @@ int sph_run_taylor_green(const sph_run_config* config, sph_diagnostics* final_ro
  * solenoidal initial field with E(k) ~ k^4 exp(-2 (k/4)^2), scaled to kinetic
  * energy `energy`. Members of equal size share every pass launch, sizes up
  * to 128 can share one launch per pass (sph_ensemble_launch_mode), and a
- * step is one hipGraph.
+ * step is one hipGraph (launch structure 2, cooperative, runs uncaptured).
  * Each member's state follows the same RK3 stage as TaylorGreenSolver::step.
  * ------------------------------------------------------------------------ */
 typedef struct sph_ensemble_s* sph_ensemble_t;
