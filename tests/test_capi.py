@@ -33,6 +33,13 @@ def test_library_exports_every_declared_symbol():
         assert name in bound, f"{name} has no Python binding"
 
 
+def test_integration_guide_names_every_entry_point():
+    """The binding guide a maintainer follows names every declared call."""
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    missing = [name for name in declared_functions() if not re.search(rf"\b{name}\b", guide)]
+    assert not missing, f"INTEGRATION.md does not name {missing}"
+
+
 def test_supported_lengths():
     lib = sph.load_library()
     for n in (8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024):
