@@ -40,6 +40,25 @@ def test_integration_guide_names_every_entry_point():
     assert not missing, f"INTEGRATION.md does not name {missing}"
 
 
+def test_integration_guide_c_examples_compile(tmp_path):
+    """The guide's C examples are statement blocks against the header: each
+    compiles as a function body (C11, every warning an error)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```c\n(.*?)```", guide, flags=re.S)
+    assert blocks
+    for i, body in enumerate(blocks):
+        src = tmp_path / f"example_{i}.c"
+        src.write_text('#include "spectral_hip.h"\nvoid example(void) {\n' + body + "}\n")
+        r = subprocess.run([gcc, "-std=c11", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                            str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
 def test_supported_lengths():
     lib = sph.load_library()
     for n in (8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024):
