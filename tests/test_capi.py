@@ -59,6 +59,36 @@ def test_integration_guide_c_examples_compile(tmp_path):
         assert r.returncode == 0, r.stderr
 
 
+def test_integration_guide_cpp_examples_compile(tmp_path):
+    """The guide's two C++ examples against include/spectral_hip.hpp with a
+    plain host compiler: the backend switch as a translation unit, the step
+    API as a function body (its free names t, init_seconds, run_seconds as
+    parameters)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```cpp\n(.*?)```", guide, flags=re.S)
+    assert len(blocks) == 2, "update this test with the guide's C++ examples"
+    backend, step = blocks
+    lines = step.splitlines()
+    includes = [l for l in lines if l.startswith("#include")]
+    body = [l for l in lines if not l.startswith("#include")]
+    sources = {
+        "backend.cpp": backend,
+        "step.cpp": "\n".join(includes) + "\nvoid example(double t, double init_seconds, double run_seconds) {\n"
+        + "\n".join(body) + "\n}\n",
+    }
+    for name, text in sources.items():
+        src = tmp_path / name
+        src.write_text(text)
+        r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, f"{name}: {r.stderr}"
+
+
 def test_supported_lengths():
     lib = sph.load_library()
     for n in (8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024):
